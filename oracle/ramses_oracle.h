@@ -1,0 +1,105 @@
+/*
+ * CPU oracle for the fine-level Godunov hydro step of RAMSES II.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * CPU path (hydro/umuscl.f90, hydro/godunov_utils.f90, hydro/godunov_fine.f90,
+ * hydro/courant_fine.f90, hydro/interpol_hydro.f90, amr/nbors_utils.f90,
+ * amr/boundaries.f90), written in the reference's expression order so that it
+ * agrees bit for bit with the Fortran when both are compiled without FMA
+ * contraction.  Only tests/, __graft_entry__.smoke() and the cpu_baseline leg
+ * of bench.py may load it, and only as the checker or the timed CPU baseline;
+ * the product path (ramses2_amd) never calls it.
+ *
+ * Array layouts follow the reference exactly:
+ *   uold/unew : real(8) u(1:twotondim, 1:nvar, 1:ngrid), column-major
+ *   unsplit   : uin(iu1:iu2, ju1:ju2, ku1:ku2, 1:nvar) with iu1=-1, iu2=nn+2
+ *               flux(if1:if2, jf1:jf2, kf1:kf2, 1:nvar, 1:ndim), if1=1, if2=nn+1
+ */
+#ifndef RAMSES_ORACLE_H
+#define RAMSES_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORA_NVAR 5
+
+/* Solver ids: hydro/hydro_parameters.f90:30-32 */
+#define ORA_LLF 1
+#define ORA_HLL 2
+#define ORA_HLLC 3
+
+/* hydro_params_t, amr/amr_commons.f90:690-702 (hydro subset) */
+typedef struct {
+    double gamma, smallr, smallc, courant_factor, smax_cap, difmag;
+    int slope_type, riemann;
+} ora_params_t;
+
+/* Mesh as seen by godfine1 / courant_fine.  Octs are numbered 1..ngrid
+ * (Fortran numbering); arrays are 0-based in C, so oct i lives at [i-1].
+ * Boundary types per dimension and side follow &BOUNDARY_PARAMS bound_type:
+ * 1 reflexive, 2 zero gradient (amr/boundaries.f90:39-165).               */
+typedef struct {
+    int ndim, nvar, ngrid;
+    const int32_t *ckey;      /* [ngrid][3]                                 */
+    const int32_t *lev;       /* [ngrid]                                    */
+    const uint8_t *refined;   /* [ngrid][8]                                 */
+    double *uold, *unew;      /* [ngrid][nvar][twotondim]                   */
+    int levelmin, nlevelmax;
+    const int32_t *head;      /* head(l), 1-based, l=1..nlevelmax -> [l-1] */
+    const int32_t *noct;      /* noct(l)                                    */
+    double boxlen;
+    int periodic[3];
+    int bound_type[3][2];     /* [dim][0=low side,1=high side]             */
+    int interpol_var, interpol_type;
+    double constant_gravity[3];
+} ora_mesh_t;
+
+/* Riemann solvers, hydro/godunov_utils.f90:338-857.  q[5] = (rho,u_n,u_t1,u_t2,p);
+ * f[6] = fluxes + internal-energy flux.                                       */
+void ora_riemann_llf(const double *ql, const double *qr, double *f, const ora_params_t *p);
+void ora_riemann_hll(const double *ql, const double *qr, double *f, const ora_params_t *p);
+void ora_riemann_hllc(const double *ql, const double *qr, double *f, const ora_params_t *p);
+
+/* unsplit, hydro/umuscl.f90:20-292, for ndim 1 or 3 and nn = 2*nsubgrid.
+ * p->smax_cap is overwritten as in the reference (:99).                      */
+void ora_unsplit(int ndim, int nn, const double *uin, const double *gin,
+                 double *flux, double dx, double dt, ora_params_t *p);
+
+/* cmpdt, hydro/godunov_utils.f90:228-337 (uu is modified as in the reference) */
+void ora_cmpdt(int ndim, double *uu, const double *gg, double dx,
+               const ora_params_t *p, double *dt);
+
+/* interpol_hydro, hydro/interpol_hydro.f90:5-113 (u1 modified in place when
+ * interpol_var==1, as in the reference).  u1[(2*ndim+1)][nvar] column-major
+ * as u1(0:twondim,1:nvar); u2(1:twotondim,1:nvar).                          */
+void ora_interpol_hydro(int ndim, double *u1, double *u2, int interpol_var,
+                        int interpol_type, double smallr);
+
+/* godunov_fine for one level: godfine1 over all octs of the level
+ * (hydro/godunov_fine.f90:43-79,244-1342).  unew is updated in place.
+ * Returns 0, or a nonzero code when a parent cell is missing.               */
+int ora_godunov_fine(const ora_mesh_t *m, int ilevel, double dt, const ora_params_t *p);
+
+/* courant_fine, hydro/courant_fine.f90:57-168.  out = {mass, ekin, eint, emag, dt}.
+ * nthreads>1 splits the octs into contiguous chunks whose partial sums are
+ * combined in chunk order (deterministic for a given nthreads).             */
+void ora_courant_fine(const ora_mesh_t *m, int ilevel, const ora_params_t *p,
+                      int nthreads, double *out);
+
+/* set_unew / set_uold, hydro/godunov_fine.f90:114-148,183-214 */
+void ora_set_unew(const ora_mesh_t *m, int ilevel);
+void ora_set_uold(const ora_mesh_t *m, int ilevel);
+
+/* Uniform-level fast path for the CPU baseline: same arithmetic as
+ * ora_godunov_fine on a periodic uniform level whose octs are addressed
+ * through a dense (ix,iy,iz) -> oct map, run with OpenMP over octs.          */
+int ora_godunov_uniform_omp(const ora_mesh_t *m, int ilevel, double dt,
+                            const ora_params_t *p, const int32_t *octmap,
+                            int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

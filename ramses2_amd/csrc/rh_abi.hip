@@ -1,0 +1,693 @@
+// C-ABI of the gfx950 hydro library (include/ramses2_hip.h): device state,
+// device hash, neighbour / brick tables and the drop-in rh_* entry points.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/ramses2_hip.h"
+#include "rh_godunov.h"
+#include "rh_state.h"
+
+namespace rh {
+
+static State g_state;
+State &st() { return g_state; }
+void set_error(const std::string &msg) { g_state.last_error = msg; }
+
+static int fail(int code, const std::string &msg)
+{
+    g_state.last_error = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------------------
+// Device hash: open addressing, linear probing, 64-bit keys.
+// key = key_off(lev) + ix + iy*nx + iz*nx^2 (gpu/gpu_hash.cuf:240-245), nx = 2^(lev-1).
+// ---------------------------------------------------------------------------
+static constexpr uint64_t kEmpty = ~0ull;
+
+__device__ __forceinline__ uint64_t hmix(uint64_t k)
+{
+    k ^= k >> 31;
+    k *= 0x7fb5d329728ea185ull;
+    k ^= k >> 27;
+    k *= 0x81dadef4bc2dd44dull;
+    k ^= k >> 33;
+    return k;
+}
+
+__device__ __forceinline__ uint64_t oct_key(int64_t koff, int nx, int ndim, int ix, int iy, int iz)
+{
+    uint64_t k = (uint64_t)koff + (uint64_t)ix;
+    if (ndim > 1) k += (uint64_t)iy * (uint64_t)nx;
+    if (ndim > 2) k += (uint64_t)iz * (uint64_t)nx * (uint64_t)nx;
+    return k;
+}
+
+__device__ __forceinline__ int hash_get(const uint64_t *__restrict__ keys, const int32_t *__restrict__ vals,
+                                        uint64_t cap, uint64_t key)
+{
+    uint64_t h = hmix(key) & (cap - 1);
+    for (uint64_t n = 0; n < cap; n++) {
+        const uint64_t k = keys[h];
+        if (k == key) return vals[h];
+        if (k == kEmpty) return 0;
+        h = (h + 1) & (cap - 1);
+    }
+    return 0;
+}
+
+struct LevelTab {
+    int64_t key_off[64];
+    int32_t nx[64];
+};
+
+__global__ void k_hash_insert(const int32_t *__restrict__ ckey, const int32_t *__restrict__ lev, int head, int n,
+                              int ndim, LevelTab lt, int nlev, uint64_t *keys, int32_t *vals, uint64_t cap,
+                              int *overflow)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = head - 1 + i;
+    const int l = lev[o];
+    if (l < 1 || l > nlev) {
+        atomicOr(overflow, 2);
+        return;
+    }
+    const uint64_t key = oct_key(lt.key_off[l - 1], lt.nx[l - 1], ndim, ckey[3 * o], ckey[3 * o + 1], ckey[3 * o + 2]);
+    uint64_t h = hmix(key) & (cap - 1);
+    for (uint64_t p = 0; p < cap; p++) {
+        unsigned long long prev = atomicCAS((unsigned long long *)&keys[h], (unsigned long long)kEmpty,
+                                            (unsigned long long)key);
+        if (prev == kEmpty || prev == key) {
+            vals[h] = o + 1;
+            return;
+        }
+        h = (h + 1) & (cap - 1);
+    }
+    atomicOr(overflow, 1);
+}
+
+// Resolve the oct at (possibly out-of-domain) key k of a level.  Returns the
+// oct index, -(reference oct) for a boundary ghost, 0 if absent.  Periodic
+// directions wrap modulo the box (hydro/godunov_fine.f90:463-468).
+struct ResolveCtx {
+    const uint64_t *keys;
+    const int32_t *vals;
+    uint64_t cap;
+    int64_t koff;
+    int nx, ndim;
+    int boxmin[3], boxmax[3], periodic[3];
+};
+
+__device__ __forceinline__ int resolve_oct(const ResolveCtx &c, int k0, int k1, int k2)
+{
+    int k[3] = {k0, k1, k2};
+    for (int d = 0; d < c.ndim; d++) {
+        if (c.periodic[d]) {
+            const int n = c.boxmax[d] - c.boxmin[d];
+            int r = (k[d] - c.boxmin[d]) % n;
+            if (r < 0) r += n;
+            k[d] = c.boxmin[d] + r;
+        }
+    }
+    int out_dim = -1, side = 0;
+    for (int d = c.ndim - 1; d >= 0; d--) {
+        if (k[d] < c.boxmin[d]) { out_dim = d; side = 0; }
+        else if (k[d] >= c.boxmax[d]) { out_dim = d; side = 1; }
+    }
+    if (out_dim < 0) return hash_get(c.keys, c.vals, c.cap, oct_key(c.koff, c.nx, c.ndim, k[0], k[1], k[2]));
+    // boundary ghost: the oct one step back into the domain along out_dim
+    k[out_dim] += side == 0 ? 1 : -1;
+    for (int d = 0; d < c.ndim; d++)
+        if (k[d] < c.boxmin[d] || k[d] >= c.boxmax[d]) return 0;
+    const int r = hash_get(c.keys, c.vals, c.cap, oct_key(c.koff, c.nx, c.ndim, k[0], k[1], k[2]));
+    return r > 0 ? -r : 0;
+}
+
+// 3^ndim neighbour slots of each oct of the level (slot 1+i+3j+9k, 0-based here).
+__global__ void k_build_nbor(const int32_t *__restrict__ ckey, int head, int noct, ResolveCtx c, int nslot,
+                             int32_t *__restrict__ nbor)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)noct * nslot) return;
+    const int o = head - 1 + (int)(i / nslot);
+    const int s = (int)(i % nslot);
+    const int di = s % 3 - 1, dj = (s / 3) % 3 - 1, dk = s / 9 - 1;
+    nbor[i] = resolve_oct(c, ckey[3 * o] + di, ckey[3 * o + 1] + dj, ckey[3 * o + 2] + dk);
+}
+
+// Brick windows: (bx+2)(by+2)(bz+2) slots per brick, slot (ox,oy,oz) = oct at
+// org + (ox-1, oy-1, oz-1).
+__global__ void k_build_bricks(const int32_t *__restrict__ org, int nbricks, int bx, int by, int bz, ResolveCtx c,
+                               int32_t *__restrict__ win)
+{
+    const int wx = bx + 2, wy = by + 2, wz = bz + 2, nw = wx * wy * wz;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)nbricks * nw) return;
+    const int b = (int)(i / nw), s = (int)(i % nw);
+    const int ox = s % wx, oy = (s / wx) % wy, oz = s / (wx * wy);
+    win[i] = resolve_oct(c, org[3 * b] + ox - 1, org[3 * b + 1] + oy - 1, org[3 * b + 2] + oz - 1);
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+static int ensure_init()
+{
+    if (!g_state.inited) return fail(RH_ERR_STATE, "rh_init has not been called");
+    return 0;
+}
+
+static int ensure_mesh()
+{
+    if (int r = ensure_init()) return r;
+    if (!g_state.uold) return fail(RH_ERR_STATE, "rh_alloc_amr has not been called");
+    return 0;
+}
+
+static void free_level_tables()
+{
+    for (auto &kv : g_state.bricks) {
+        (void)hipFree(kv.second.win);
+        (void)hipFree(kv.second.org);
+    }
+    g_state.bricks.clear();
+    for (auto &kv : g_state.nbors) (void)hipFree(kv.second.nbor);
+    g_state.nbors.clear();
+}
+
+static int level_of_range(int head_idx, int num_octs, int &ilevel)
+{
+    State &s = g_state;
+    if (head_idx < 1 || num_octs < 0 || (int64_t)head_idx - 1 + num_octs > s.ngridtot)
+        return fail(RH_ERR_ARG, "oct range out of bounds");
+    if (num_octs == 0) {
+        ilevel = 0;
+        return 0;
+    }
+    ilevel = s.h_lev[head_idx - 1];
+    if (ilevel < 1 || ilevel > s.nlevelmax) return fail(RH_ERR_MESH, "oct level outside the level parameters");
+    return 0;
+}
+
+static ResolveCtx make_resolve(int ilevel, const int32_t box_min[3], const int32_t box_max[3],
+                               const int32_t periodic[3])
+{
+    State &s = g_state;
+    ResolveCtx c;
+    c.keys = s.hkeys;
+    c.vals = s.hvals;
+    c.cap = (uint64_t)s.hash_cap;
+    c.koff = s.key_off[ilevel - 1];
+    c.nx = s.ckey_max[ilevel - 1];
+    c.ndim = s.ndim;
+    for (int d = 0; d < 3; d++) {
+        c.boxmin[d] = d < s.ndim ? box_min[d] : 0;
+        c.boxmax[d] = d < s.ndim ? box_max[d] : 1;
+        c.periodic[d] = d < s.ndim ? periodic[d] : 1;
+    }
+    return c;
+}
+
+}  // namespace rh
+
+using namespace rh;
+
+// ---------------------------------------------------------------------------
+// Entry points
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *rh_last_error(void) { return g_state.last_error.c_str(); }
+
+int rh_init(void)
+{
+    State &s = g_state;
+    if (s.inited) return 0;
+    int n = 0;
+    RH_HIP_CHECK(hipGetDeviceCount(&n));
+    if (n < 1) return fail(RH_ERR_HIP, "no HIP device visible");
+    RH_HIP_CHECK(hipGetDevice(&s.device));
+    RH_HIP_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    RH_HIP_CHECK(hipMalloc(&s.red_out, 8 * sizeof(double)));
+    RH_HIP_CHECK(hipHostMalloc(&s.h_red, 8 * sizeof(double), hipHostMallocDefault));
+    s.red_cap = 4096;
+    RH_HIP_CHECK(hipMalloc(&s.red_part, (size_t)s.red_cap * 4 * sizeof(double)));
+    RH_HIP_CHECK(hipMalloc(&s.dt_dev, 2 * sizeof(double)));
+    RH_HIP_CHECK(hipMalloc(&s.t_dev, 2 * sizeof(double)));
+    RH_HIP_CHECK(hipEventCreate(&s.ev0));
+    RH_HIP_CHECK(hipEventCreate(&s.ev1));
+    s.inited = true;
+    s.last_error.clear();
+    return 0;
+}
+
+int rh_finalize(void)
+{
+    State &s = g_state;
+    if (!s.inited) return 0;
+    (void)hipStreamSynchronize(s.stream);
+    free_level_tables();
+    (void)hipFree(s.uold);
+    (void)hipFree(s.unew);
+    (void)hipFree(s.ckey);
+    (void)hipFree(s.lev);
+    (void)hipFree(s.refined);
+    (void)hipFree(s.hkeys);
+    (void)hipFree(s.hvals);
+    (void)hipFree(s.red_part);
+    (void)hipFree(s.red_out);
+    (void)hipHostFree(s.h_red);
+    (void)hipFree(s.dt_dev);
+    (void)hipFree(s.t_dev);
+    (void)hipEventDestroy(s.ev0);
+    (void)hipEventDestroy(s.ev1);
+    (void)hipStreamDestroy(s.stream);
+    s = State();
+    return 0;
+}
+
+int rh_alloc_amr(int ngridmax, int ncachemax, int nvar, int twotondim, int hash_size)
+{
+    if (int r = ensure_init()) return r;
+    State &s = g_state;
+    if (nvar != 5) return fail(RH_ERR_ARG, "nvar must be 5 (hydro, no passive scalars)");
+    if (twotondim != 2 && twotondim != 8) return fail(RH_ERR_ARG, "twotondim must be 2 (1D) or 8 (3D)");
+    if (ngridmax < 1 || ncachemax < 0 || (int64_t)ngridmax + ncachemax > (1 << 29))
+        return fail(RH_ERR_ARG, "ngridmax+ncachemax must be in [1, 2^29]");
+    if (hash_size < 1) return fail(RH_ERR_ARG, "hash_size must be positive");
+    (void)hipStreamSynchronize(s.stream);
+    free_level_tables();
+    (void)hipFree(s.uold);
+    (void)hipFree(s.unew);
+    (void)hipFree(s.ckey);
+    (void)hipFree(s.lev);
+    (void)hipFree(s.refined);
+    (void)hipFree(s.hkeys);
+    (void)hipFree(s.hvals);
+    s.ndim = twotondim == 8 ? 3 : 1;
+    s.ttd = twotondim;
+    s.nvar = nvar;
+    s.ngridmax = ngridmax;
+    s.ncachemax = ncachemax;
+    s.ngridtot = ngridmax + ncachemax;
+    int64_t cap = 1;
+    while (cap < hash_size) cap <<= 1;
+    s.hash_cap = cap;
+    const size_t nd = (size_t)s.ngridtot * nvar * twotondim;
+    if (hipMalloc(&s.uold, nd * sizeof(double)) != hipSuccess || hipMalloc(&s.unew, nd * sizeof(double)) != hipSuccess ||
+        hipMalloc(&s.ckey, (size_t)s.ngridtot * 3 * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&s.lev, (size_t)s.ngridtot * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&s.refined, (size_t)s.ngridtot * 8) != hipSuccess ||
+        hipMalloc(&s.hkeys, (size_t)cap * sizeof(uint64_t)) != hipSuccess ||
+        hipMalloc(&s.hvals, (size_t)cap * sizeof(int32_t)) != hipSuccess)
+        return fail(RH_ERR_NOMEM, "device allocation failed in rh_alloc_amr");
+    RH_HIP_CHECK(hipMemsetAsync(s.uold, 0, nd * sizeof(double), s.stream));
+    RH_HIP_CHECK(hipMemsetAsync(s.unew, 0, nd * sizeof(double), s.stream));
+    RH_HIP_CHECK(hipMemsetAsync(s.ckey, 0, (size_t)s.ngridtot * 3 * sizeof(int32_t), s.stream));
+    RH_HIP_CHECK(hipMemsetAsync(s.lev, 0, (size_t)s.ngridtot * sizeof(int32_t), s.stream));
+    RH_HIP_CHECK(hipMemsetAsync(s.refined, 0, (size_t)s.ngridtot * 8, s.stream));
+    RH_HIP_CHECK(hipMemsetAsync(s.hkeys, 0xFF, (size_t)cap * sizeof(uint64_t), s.stream));
+    RH_HIP_CHECK(hipMemsetAsync(s.hvals, 0, (size_t)cap * sizeof(int32_t), s.stream));
+    s.h_ckey.assign((size_t)s.ngridtot * 3, 0);
+    s.h_lev.assign((size_t)s.ngridtot, 0);
+    RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int rh_upload_level_params(const int32_t *ckey_max, const int64_t *key_off, const int32_t *box_ckey_min,
+                           const int32_t *box_ckey_max, const int32_t periodic[3], int nlevelmax)
+{
+    if (int r = ensure_init()) return r;
+    State &s = g_state;
+    if (nlevelmax < 1 || nlevelmax > 63) return fail(RH_ERR_ARG, "nlevelmax must be in [1, 63]");
+    if (!ckey_max || !key_off || !box_ckey_min || !box_ckey_max || !periodic)
+        return fail(RH_ERR_ARG, "null level parameter array");
+    s.nlevelmax = nlevelmax;
+    s.ckey_max.assign(ckey_max, ckey_max + nlevelmax + 1);
+    s.key_off.assign(key_off, key_off + nlevelmax + 1);
+    s.box_min.assign(box_ckey_min, box_ckey_min + 3 * (nlevelmax + 1));
+    s.box_max.assign(box_ckey_max, box_ckey_max + 3 * (nlevelmax + 1));
+    for (int d = 0; d < 3; d++) s.periodic[d] = periodic[d] ? 1 : 0;
+    return 0;
+}
+
+int rh_set_boundaries(const int32_t bound_type[6])
+{
+    if (!bound_type) return fail(RH_ERR_ARG, "null bound_type");
+    for (int d = 0; d < 3; d++)
+        for (int side = 0; side < 2; side++) {
+            const int t = bound_type[2 * d + side];
+            if (t != 1 && t != 2) return fail(RH_ERR_ARG, "bound_type must be 1 (reflexive) or 2 (zero gradient)");
+            g_state.btype[d][side] = t;
+        }
+    return 0;
+}
+
+int rh_set_grid_device(const double *uold, const double *unew, const void *bold, const void *grid, int ngridmax,
+                       int nvar, int twotondim)
+{
+    if (int r = ensure_mesh()) return r;
+    State &s = g_state;
+    if (bold) return fail(RH_ERR_ARG, "bold must be NULL (no MHD)");
+    if (ngridmax < 0 || ngridmax > s.ngridtot || nvar != s.nvar || twotondim != s.ttd)
+        return fail(RH_ERR_ARG, "rh_set_grid_device: sizes differ from rh_alloc_amr");
+    const size_t per = (size_t)nvar * twotondim;
+    if (uold) RH_HIP_CHECK(hipMemcpyAsync(s.uold, uold, per * ngridmax * sizeof(double), hipMemcpyHostToDevice, s.stream));
+    if (unew) RH_HIP_CHECK(hipMemcpyAsync(s.unew, unew, per * ngridmax * sizeof(double), hipMemcpyHostToDevice, s.stream));
+    if (grid) {
+        // unpack the reference oct records into SoA arrays
+        const size_t rec = s.ndim == 3 ? 32 : 24;
+        const size_t off_ckey = 8, off_ref = s.ndim == 3 ? 20 : 12, off_lev = s.ndim == 3 ? 28 : 16;
+        std::vector<uint8_t> refb((size_t)ngridmax * 8, 0);
+        const uint8_t *g = (const uint8_t *)grid;
+        for (int i = 0; i < ngridmax; i++) {
+            const uint8_t *r = g + rec * i;
+            int32_t ck[3] = {0, 0, 0};
+            memcpy(ck, r + off_ckey, sizeof(int32_t) * s.ndim);
+            memcpy(&s.h_ckey[3 * (size_t)i], ck, sizeof(ck));
+            memcpy(&s.h_lev[i], r + off_lev, sizeof(int32_t));
+            memcpy(&refb[8 * (size_t)i], r + off_ref, s.ttd);
+        }
+        RH_HIP_CHECK(hipMemcpyAsync(s.ckey, s.h_ckey.data(), (size_t)ngridmax * 3 * sizeof(int32_t), hipMemcpyHostToDevice, s.stream));
+        RH_HIP_CHECK(hipMemcpyAsync(s.lev, s.h_lev.data(), (size_t)ngridmax * sizeof(int32_t), hipMemcpyHostToDevice, s.stream));
+        RH_HIP_CHECK(hipMemcpyAsync(s.refined, refb.data(), (size_t)ngridmax * 8, hipMemcpyHostToDevice, s.stream));
+        RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+        // mesh changed: tables must be rebuilt
+        free_level_tables();
+        RH_HIP_CHECK(hipMemsetAsync(s.hkeys, 0xFF, (size_t)s.hash_cap * sizeof(uint64_t), s.stream));
+    }
+    RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int rh_transfer_grid_host(double *uold, void *bold, int ngridmax, int nvar, int twotondim)
+{
+    if (int r = ensure_mesh()) return r;
+    State &s = g_state;
+    (void)bold;
+    if (!uold || ngridmax < 0 || ngridmax > s.ngridtot || nvar != s.nvar || twotondim != s.ttd)
+        return fail(RH_ERR_ARG, "rh_transfer_grid_host: bad arguments");
+    RH_HIP_CHECK(hipMemcpyAsync(uold, s.uold, (size_t)nvar * twotondim * ngridmax * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int rh_transfer_unew_host(double *unew, int ngridmax, int nvar, int twotondim)
+{
+    if (int r = ensure_mesh()) return r;
+    State &s = g_state;
+    if (!unew || ngridmax < 0 || ngridmax > s.ngridtot || nvar != s.nvar || twotondim != s.ttd)
+        return fail(RH_ERR_ARG, "rh_transfer_unew_host: bad arguments");
+    RH_HIP_CHECK(hipMemcpyAsync(unew, s.unew, (size_t)nvar * twotondim * ngridmax * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int rh_insert_hash_all(int head_idx, int num_octs, int hash_size)
+{
+    if (int r = ensure_mesh()) return r;
+    State &s = g_state;
+    (void)hash_size;
+    if (s.nlevelmax < 1) return fail(RH_ERR_STATE, "rh_upload_level_params has not been called");
+    if (head_idx < 1 || num_octs < 0 || (int64_t)head_idx - 1 + num_octs > s.ngridtot)
+        return fail(RH_ERR_ARG, "rh_insert_hash_all: oct range out of bounds");
+    if (num_octs == 0) return 0;
+    LevelTab lt;
+    for (int l = 0; l < 64; l++) {
+        lt.key_off[l] = l < s.nlevelmax ? s.key_off[l] : 0;
+        lt.nx[l] = l < s.nlevelmax ? s.ckey_max[l] : 0;
+    }
+    int *d_over = nullptr;
+    RH_HIP_CHECK(hipMalloc(&d_over, sizeof(int)));
+    RH_HIP_CHECK(hipMemsetAsync(d_over, 0, sizeof(int), s.stream));
+    hipLaunchKernelGGL(k_hash_insert, dim3((num_octs + 255) / 256), dim3(256), 0, s.stream, s.ckey, s.lev, head_idx,
+                       num_octs, s.ndim, lt, s.nlevelmax, s.hkeys, s.hvals, (uint64_t)s.hash_cap, d_over);
+    RH_HIP_CHECK(hipGetLastError());
+    int over = 0;
+    RH_HIP_CHECK(hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+    RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+    (void)hipFree(d_over);
+    if (over & 1) return fail(RH_ERR_MESH, "device hash table full");
+    if (over & 2) return fail(RH_ERR_MESH, "oct with a level outside [1, nlevelmax]");
+    return 0;
+}
+
+int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int64_t key_off_l,
+                  const int32_t box_min[3], const int32_t box_max[3], const int32_t periodic[3])
+{
+    if (int r = ensure_mesh()) return r;
+    State &s = g_state;
+    (void)hash_size;
+    (void)ckey_max_l;
+    (void)key_off_l;
+    if (!box_min || !box_max || !periodic) return fail(RH_ERR_ARG, "null box arrays");
+    int ilevel = 0;
+    if (int r = level_of_range(head_idx, num_octs, ilevel)) return r;
+    if (num_octs == 0) return 0;
+    for (int i = head_idx - 1; i < head_idx - 1 + num_octs; i++)
+        if (s.h_lev[i] != ilevel) return fail(RH_ERR_MESH, "octs of the range are not all on one level");
+    ResolveCtx c = make_resolve(ilevel, box_min, box_max, periodic);
+
+    // per-oct neighbour table (3^ndim slots)
+    {
+        auto it = s.nbors.find(ilevel);
+        if (it != s.nbors.end()) {
+            (void)hipFree(it->second.nbor);
+            s.nbors.erase(it);
+        }
+        LevelNbor ln;
+        ln.ilevel = ilevel;
+        ln.head = head_idx;
+        ln.noct = num_octs;
+        const int nslot = s.ndim == 3 ? 27 : 3;
+        RH_HIP_CHECK(hipMalloc(&ln.nbor, (size_t)num_octs * nslot * sizeof(int32_t)));
+        const long n = (long)num_octs * nslot;
+        hipLaunchKernelGGL(k_build_nbor, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s.stream, s.ckey, head_idx,
+                           num_octs, c, nslot, ln.nbor);
+        RH_HIP_CHECK(hipGetLastError());
+        for (int d = 0; d < 3; d++) {
+            ln.boxmin[d] = c.boxmin[d];
+            ln.boxmax[d] = c.boxmax[d];
+        }
+        s.nbors[ilevel] = ln;
+    }
+    if (s.ndim == 3) {
+        auto it = s.bricks.find(ilevel);
+        if (it != s.bricks.end()) {
+            (void)hipFree(it->second.win);
+            (void)hipFree(it->second.org);
+            s.bricks.erase(it);
+        }
+        LevelBricks lb;
+        lb.ilevel = ilevel;
+        lb.head = head_idx;
+        lb.noct = num_octs;
+        lb.bx = s.brick_bx;
+        lb.by = s.brick_by;
+        lb.bz = s.brick_bz;
+        // enumerate the bricks that hold at least one oct of the level
+        std::vector<int64_t> keys;
+        keys.reserve(num_octs);
+        const int64_t nbx = (c.boxmax[0] - c.boxmin[0] + lb.bx - 1) / lb.bx;
+        const int64_t nby = (c.boxmax[1] - c.boxmin[1] + lb.by - 1) / lb.by;
+        for (int i = head_idx - 1; i < head_idx - 1 + num_octs; i++) {
+            const int64_t bx = (s.h_ckey[3 * (size_t)i] - c.boxmin[0]) / lb.bx;
+            const int64_t by = (s.h_ckey[3 * (size_t)i + 1] - c.boxmin[1]) / lb.by;
+            const int64_t bz = (s.h_ckey[3 * (size_t)i + 2] - c.boxmin[2]) / lb.bz;
+            keys.push_back((bz * nby + by) * nbx + bx);
+        }
+        std::sort(keys.begin(), keys.end());
+        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+        lb.nbricks = (int)keys.size();
+        std::vector<int32_t> org(3 * keys.size());
+        for (size_t b = 0; b < keys.size(); b++) {
+            const int64_t k = keys[b];
+            org[3 * b + 0] = c.boxmin[0] + (int32_t)(k % nbx) * lb.bx;
+            org[3 * b + 1] = c.boxmin[1] + (int32_t)((k / nbx) % nby) * lb.by;
+            org[3 * b + 2] = c.boxmin[2] + (int32_t)(k / (nbx * nby)) * lb.bz;
+        }
+        const int nw = (lb.bx + 2) * (lb.by + 2) * (lb.bz + 2);
+        RH_HIP_CHECK(hipMalloc(&lb.org, org.size() * sizeof(int32_t)));
+        RH_HIP_CHECK(hipMalloc(&lb.win, (size_t)lb.nbricks * nw * sizeof(int32_t)));
+        RH_HIP_CHECK(hipMemcpyAsync(lb.org, org.data(), org.size() * sizeof(int32_t), hipMemcpyHostToDevice, s.stream));
+        const long n = (long)lb.nbricks * nw;
+        hipLaunchKernelGGL(k_build_bricks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s.stream, lb.org,
+                           lb.nbricks, lb.bx, lb.by, lb.bz, c, lb.win);
+        RH_HIP_CHECK(hipGetLastError());
+        for (int d = 0; d < 3; d++) {
+            lb.boxmin[d] = c.boxmin[d];
+            lb.boxmax[d] = c.boxmax[d];
+        }
+        s.bricks[ilevel] = lb;
+        // every oct of the level must be found in its own brick window
+        std::vector<int32_t> hwin((size_t)n);
+        RH_HIP_CHECK(hipMemcpyAsync(hwin.data(), lb.win, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream));
+        RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+        long found = 0;
+        const int wx = lb.bx + 2, wy = lb.by + 2;
+        for (int b = 0; b < lb.nbricks; b++)
+            for (int oz = 1; oz <= lb.bz; oz++)
+                for (int oy = 1; oy <= lb.by; oy++)
+                    for (int ox = 1; ox <= lb.bx; ox++) {
+                        if (org[3 * b] + ox - 1 >= c.boxmax[0] || org[3 * b + 1] + oy - 1 >= c.boxmax[1] ||
+                            org[3 * b + 2] + oz - 1 >= c.boxmax[2])
+                            continue;
+                        const int v = hwin[(size_t)b * nw + (oz * wy + oy) * wx + ox];
+                        if (v >= head_idx && v < head_idx + num_octs) found++;
+                    }
+        if (found != num_octs) return fail(RH_ERR_MESH, "octs of the level missing from the device hash");
+    }
+    RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+int rh_set_unew(int head_idx, int num_octs)
+{
+    if (int r = ensure_mesh()) return r;
+    State &s = g_state;
+    if (head_idx < 1 || num_octs < 0 || (int64_t)head_idx - 1 + num_octs > s.ngridtot)
+        return fail(RH_ERR_ARG, "rh_set_unew: oct range out of bounds");
+    const size_t per = (size_t)s.nvar * s.ttd;
+    RH_HIP_CHECK(hipMemcpyAsync(s.unew + per * (head_idx - 1), s.uold + per * (head_idx - 1),
+                                per * num_octs * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+    return 0;
+}
+
+int rh_set_uold(int head_idx, int num_octs)
+{
+    if (int r = ensure_mesh()) return r;
+    State &s = g_state;
+    if (head_idx < 1 || num_octs < 0 || (int64_t)head_idx - 1 + num_octs > s.ngridtot)
+        return fail(RH_ERR_ARG, "rh_set_uold: oct range out of bounds");
+    const size_t per = (size_t)s.nvar * s.ttd;
+    RH_HIP_CHECK(hipMemcpyAsync(s.uold + per * (head_idx - 1), s.unew + per * (head_idx - 1),
+                                per * num_octs * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+    return 0;
+}
+
+int rh_cmpdt(int head_idx, int num_octs, double dx, double gamma, double smallr, double smallc2, double courant_factor,
+             const double g[3], double *mass, double *ekin, double *eint, double *emag, double *dt)
+{
+    if (int r = ensure_mesh()) return r;
+    State &s = g_state;
+    if (head_idx < 1 || num_octs < 0 || (int64_t)head_idx - 1 + num_octs > s.ngridtot)
+        return fail(RH_ERR_ARG, "rh_cmpdt: oct range out of bounds");
+    if (!(dx > 0) || !(smallc2 > 0) || !(gamma > 1)) return fail(RH_ERR_ARG, "rh_cmpdt: bad physical parameters");
+    const double gz[3] = {0, 0, 0};
+    const double smallc = __builtin_sqrt(smallc2);
+    RH_HIP_CHECK(launch_courant(s.uold, s.refined, head_idx, num_octs, s.ndim, dx, gamma, smallr, smallc,
+                                courant_factor, g ? g : gz, s.red_part, s.red_cap, s.red_out, nullptr, s.stream));
+    RH_HIP_CHECK(hipMemcpyAsync(s.h_red, s.red_out, 5 * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+    if (mass) *mass = s.h_red[0];
+    if (ekin) *ekin = s.h_red[1];
+    if (eint) *eint = s.h_red[2];
+    if (emag) *emag = s.h_red[3];
+    if (dt) *dt = s.h_red[4];
+    return 0;
+}
+
+int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int levelmin, int levelmax, double gamma,
+               double smallr, double smallc2, double dt, double dx, double smax_cap, int slope, int riemann,
+               const double g[3])
+{
+    if (int r = ensure_mesh()) return r;
+    State &s = g_state;
+    (void)ngridmax;
+    (void)levelmax;
+    if (num_subgrids == 0) return 0;
+    if (riemann < 1 || riemann > 3) return fail(RH_ERR_ARG, "riemann must be 1 (llf), 2 (hll) or 3 (hllc)");
+    if (slope < 0 || slope > 3) return fail(RH_ERR_ARG, "slope_type must be 0..3");
+    if (!(dt > 0) || !(dx > 0)) return fail(RH_ERR_ARG, "dt and dx must be positive");
+    if (ilevel > levelmin) {
+        // coarse-fine refluxing is served by the AMR level path
+        // (rh_godunov rejects it until the ghost-oct pipeline is built)
+        return fail(RH_ERR_STATE, "rh_godunov: AMR levels (ilevel > levelmin) are not supported yet");
+    }
+    const double gz[3] = {0, 0, 0};
+    const double *gg = g ? g : gz;
+    if (s.ndim == 3) {
+        auto it = s.bricks.find(ilevel);
+        if (it == s.bricks.end() || it->second.head != head_idx || it->second.noct != num_subgrids)
+            return fail(RH_ERR_STATE, "rh_godunov: rh_build_nbor was not called for this level");
+        const LevelBricks &lb = it->second;
+        GodArgs a;
+        a.uold = s.uold;
+        a.unew = s.unew;
+        a.ubase = s.unew;
+        a.refined = s.refined;
+        a.win = lb.win;
+        a.org = lb.org;
+        a.nbricks = lb.nbricks;
+        a.head = head_idx;
+        a.noct = num_subgrids;
+        for (int d = 0; d < 3; d++) {
+            a.boxmin[d] = lb.boxmin[d];
+            a.boxmax[d] = lb.boxmax[d];
+            a.btype[d][0] = s.btype[d][0];
+            a.btype[d][1] = s.btype[d][1];
+            a.g[d] = gg[d];
+        }
+        a.dt_dev = nullptr;
+        a.dt = dt;
+        a.dx = dx;
+        a.inv_dx = 1.0 / dx;
+        a.gamma = gamma;
+        a.smallr = smallr;
+        a.smallc2 = smallc2;
+        a.courant_factor = 0.0;
+        a.smax_cap = smax_cap;
+        a.use_smax = 1;
+        a.slope = slope;
+        a.riemann = riemann;
+        a.xcd_remap = 1;
+        RH_HIP_CHECK(launch_godunov3d(a, lb.bx, lb.by, lb.bz, s.precision, s.stream));
+    } else {
+        auto it = s.nbors.find(ilevel);
+        if (it == s.nbors.end() || it->second.head != head_idx || it->second.noct != num_subgrids)
+            return fail(RH_ERR_STATE, "rh_godunov: rh_build_nbor was not called for this level");
+        const LevelNbor &ln = it->second;
+        God1Args a;
+        a.uold = s.uold;
+        a.unew = s.unew;
+        a.ubase = s.unew;
+        a.refined = s.refined;
+        a.nbor = ln.nbor;
+        a.head = head_idx;
+        a.noct = num_subgrids;
+        a.boxmin = ln.boxmin[0];
+        a.boxmax = ln.boxmax[0];
+        a.btype[0] = s.btype[0][0];
+        a.btype[1] = s.btype[0][1];
+        a.dt_dev = nullptr;
+        a.dt = dt;
+        a.dx = dx;
+        a.inv_dx = 1.0 / dx;
+        a.gamma = gamma;
+        a.smallr = smallr;
+        a.smallc2 = smallc2;
+        a.courant_factor = 0.0;
+        a.smax_cap = smax_cap;
+        a.use_smax = 1;
+        a.g = gg[0];
+        a.slope = slope;
+        a.riemann = riemann;
+        RH_HIP_CHECK(launch_godunov1d(a, s.precision, s.stream));
+    }
+    return 0;
+}
+
+int rh_device_sync(void)
+{
+    if (int r = ensure_init()) return r;
+    RH_HIP_CHECK(hipStreamSynchronize(g_state.stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,302 @@
+// Extensions beyond the drop-in ABI (include/ramses2_hip_ext.h): precision
+// switch, brick shape, event timer, and device-resident time stepping of a
+// single-level mesh with the Courant dt kept on the device (optionally replayed
+// as a captured hipGraph).
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/ramses2_hip.h"
+#include "../../include/ramses2_hip_ext.h"
+#include "rh_godunov.h"
+#include "rh_state.h"
+
+using namespace rh;
+
+namespace {
+
+int efail(int code, const char *msg)
+{
+    set_error(msg);
+    return code;
+}
+
+// The level must be the whole mesh: every allocated oct outside the range is unused.
+bool level_is_whole_mesh(int head_idx, int num_octs)
+{
+    State &s = st();
+    for (int i = 0; i < s.ngridtot; i++) {
+        const bool in = i >= head_idx - 1 && i < head_idx - 1 + num_octs;
+        if (!in && s.h_lev[i] != 0) return false;
+    }
+    return true;
+}
+
+// Fill the Godunov arguments of a 3D level in fused mode (unew = uold + dF).
+int fill_args3(int head_idx, int num_octs, int ilevel, double dx, double gamma, double smallr, double smallc2,
+               double courant_factor, const double g[3], int slope, int riemann, const double *dt_dev, double dt,
+               GodArgs &a, const LevelBricks *&lbo)
+{
+    State &s = st();
+    auto it = s.bricks.find(ilevel);
+    if (it == s.bricks.end() || it->second.head != head_idx || it->second.noct != num_octs)
+        return efail(RH_ERR_STATE, "rh_build_nbor was not called for this level");
+    const LevelBricks &lb = it->second;
+    lbo = &lb;
+    a.uold = s.uold;
+    a.unew = s.unew;
+    a.ubase = s.uold;
+    a.refined = s.refined;
+    a.win = lb.win;
+    a.org = lb.org;
+    a.nbricks = lb.nbricks;
+    a.head = head_idx;
+    a.noct = num_octs;
+    for (int d = 0; d < 3; d++) {
+        a.boxmin[d] = lb.boxmin[d];
+        a.boxmax[d] = lb.boxmax[d];
+        a.btype[d][0] = s.btype[d][0];
+        a.btype[d][1] = s.btype[d][1];
+        a.g[d] = g ? g[d] : 0.0;
+    }
+    a.dt_dev = dt_dev;
+    a.dt = dt;
+    a.dx = dx;
+    a.inv_dx = 1.0 / dx;
+    a.gamma = gamma;
+    a.smallr = smallr;
+    a.smallc2 = smallc2;
+    a.courant_factor = courant_factor;
+    a.smax_cap = 0.0;
+    a.use_smax = 0;
+    a.slope = slope;
+    a.riemann = riemann;
+    a.xcd_remap = 1;
+    return 0;
+}
+
+int fill_args1(int head_idx, int num_octs, int ilevel, double dx, double gamma, double smallr, double smallc2,
+               double courant_factor, const double g[3], int slope, int riemann, const double *dt_dev, double dt,
+               God1Args &a)
+{
+    State &s = st();
+    auto it = s.nbors.find(ilevel);
+    if (it == s.nbors.end() || it->second.head != head_idx || it->second.noct != num_octs)
+        return efail(RH_ERR_STATE, "rh_build_nbor was not called for this level");
+    const LevelNbor &ln = it->second;
+    a.uold = s.uold;
+    a.unew = s.unew;
+    a.ubase = s.uold;
+    a.refined = s.refined;
+    a.nbor = ln.nbor;
+    a.head = head_idx;
+    a.noct = num_octs;
+    a.boxmin = ln.boxmin[0];
+    a.boxmax = ln.boxmax[0];
+    a.btype[0] = s.btype[0][0];
+    a.btype[1] = s.btype[0][1];
+    a.dt_dev = dt_dev;
+    a.dt = dt;
+    a.dx = dx;
+    a.inv_dx = 1.0 / dx;
+    a.gamma = gamma;
+    a.smallr = smallr;
+    a.smallc2 = smallc2;
+    a.courant_factor = courant_factor;
+    a.smax_cap = 0.0;
+    a.use_smax = 0;
+    a.g = g ? g[0] : 0.0;
+    a.slope = slope;
+    a.riemann = riemann;
+    return 0;
+}
+
+// One fused step: courant (dt to the device) then godunov uold -> unew.
+hipError_t enqueue_step(int head_idx, int num_octs, double dx, double gamma, double smallr, double smallc2,
+                        double courant_factor, const double g[3], double dt_fixed, GodArgs *a3, God1Args *a1,
+                        const LevelBricks *lb)
+{
+    State &s = st();
+    const double gz[3] = {0, 0, 0};
+    if (dt_fixed <= 0.0) {
+        hipError_t e = launch_courant(s.uold, s.refined, head_idx, num_octs, s.ndim, dx, gamma, smallr,
+                                      __builtin_sqrt(smallc2), courant_factor, g ? g : gz, s.red_part, s.red_cap,
+                                      s.red_out, s.dt_dev, s.stream, s.t_dev);
+        if (e != hipSuccess) return e;
+    }
+    if (a3) {
+        a3->uold = s.uold;
+        a3->unew = s.unew;
+        a3->ubase = s.uold;
+        return launch_godunov3d(*a3, lb->bx, lb->by, lb->bz, s.precision, s.stream);
+    }
+    a1->uold = s.uold;
+    a1->unew = s.unew;
+    a1->ubase = s.uold;
+    return launch_godunov1d(*a1, s.precision, s.stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rh_set_precision(int precision)
+{
+    if (precision != 0 && precision != 1) return efail(RH_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
+    st().precision = precision;
+    return 0;
+}
+
+int rh_set_brick_shape(int bx, int by, int bz)
+{
+    if (!brick_shape_supported(bx, by, bz)) return efail(RH_ERR_ARG, "brick shape has no compiled kernel");
+    st().brick_bx = bx;
+    st().brick_by = by;
+    st().brick_bz = bz;
+    return 0;
+}
+
+int rh_get_brick_shape(int32_t *bx, int32_t *by, int32_t *bz)
+{
+    if (bx) *bx = st().brick_bx;
+    if (by) *by = st().brick_by;
+    if (bz) *bz = st().brick_bz;
+    return 0;
+}
+
+int rh_timer_start(void)
+{
+    State &s = st();
+    if (!s.inited) return efail(RH_ERR_STATE, "rh_init has not been called");
+    RH_HIP_CHECK(hipEventRecord(s.ev0, s.stream));
+    return 0;
+}
+
+int rh_timer_stop(float *ms)
+{
+    State &s = st();
+    if (!s.inited) return efail(RH_ERR_STATE, "rh_init has not been called");
+    RH_HIP_CHECK(hipEventRecord(s.ev1, s.stream));
+    RH_HIP_CHECK(hipEventSynchronize(s.ev1));
+    float t = 0;
+    RH_HIP_CHECK(hipEventElapsedTime(&t, s.ev0, s.ev1));
+    if (ms) *ms = t;
+    return 0;
+}
+
+int rh_level_run(int head_idx, int num_octs, int ilevel, double dx, double gamma, double smallr, double smallc2,
+                 double courant_factor, const double g[3], int slope, int riemann, double dt_fixed, int nsteps,
+                 int use_graph, double *dt_last, double *t_total, double sums[4])
+{
+    State &s = st();
+    if (!s.inited || !s.uold) return efail(RH_ERR_STATE, "library not initialised / mesh not allocated");
+    if (nsteps < 0) return efail(RH_ERR_ARG, "nsteps must be >= 0");
+    if (!level_is_whole_mesh(head_idx, num_octs))
+        return efail(RH_ERR_STATE, "rh_level_run needs a single-level mesh (use the drop-in calls for AMR)");
+    GodArgs a3;
+    God1Args a1;
+    const LevelBricks *lb = nullptr;
+    const double *dtp = dt_fixed > 0.0 ? nullptr : s.dt_dev;
+    int r = s.ndim == 3 ? fill_args3(head_idx, num_octs, ilevel, dx, gamma, smallr, smallc2, courant_factor, g, slope,
+                                     riemann, dtp, dt_fixed, a3, lb)
+                        : fill_args1(head_idx, num_octs, ilevel, dx, gamma, smallr, smallc2, courant_factor, g,
+                                     slope, riemann, dtp, dt_fixed, a1);
+    if (r) return r;
+    RH_HIP_CHECK(hipMemsetAsync(s.t_dev, 0, sizeof(double), s.stream));
+    if (use_graph && nsteps >= 2) {
+        // capture the pair of steps uold->unew, unew->uold as one graph
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+        RH_HIP_CHECK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < 2; k++) {
+            hipError_t e = enqueue_step(head_idx, num_octs, dx, gamma, smallr, smallc2, courant_factor, g, dt_fixed,
+                                        s.ndim == 3 ? &a3 : nullptr, s.ndim == 3 ? nullptr : &a1, lb);
+            std::swap(s.uold, s.unew);
+            if (e != hipSuccess) {
+                hipGraph_t g2;
+                (void)hipStreamEndCapture(s.stream, &g2);
+                set_error(hipGetErrorString(e));
+                return RH_ERR_HIP;
+            }
+        }
+        RH_HIP_CHECK(hipStreamEndCapture(s.stream, &graph));
+        RH_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        const int npairs = nsteps / 2;
+        for (int i = 0; i < npairs; i++) RH_HIP_CHECK(hipGraphLaunch(exec, s.stream));
+        RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        // after an even number of swaps the roles are back where they started;
+        // the state of the last step is in uold (the captured pair ended on a swap)
+        if (nsteps % 2) {
+            RH_HIP_CHECK(enqueue_step(head_idx, num_octs, dx, gamma, smallr, smallc2, courant_factor, g, dt_fixed,
+                                      s.ndim == 3 ? &a3 : nullptr, s.ndim == 3 ? nullptr : &a1, lb));
+            std::swap(s.uold, s.unew);
+        }
+    } else {
+        for (int k = 0; k < nsteps; k++) {
+            RH_HIP_CHECK(enqueue_step(head_idx, num_octs, dx, gamma, smallr, smallc2, courant_factor, g, dt_fixed,
+                                      s.ndim == 3 ? &a3 : nullptr, s.ndim == 3 ? nullptr : &a1, lb));
+            std::swap(s.uold, s.unew);
+        }
+    }
+    // unew mirrors uold as after set_uold
+    const size_t per = (size_t)s.nvar * s.ttd;
+    RH_HIP_CHECK(hipMemcpyAsync(s.unew + per * (head_idx - 1), s.uold + per * (head_idx - 1),
+                                per * num_octs * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+    double h[2] = {0, 0};
+    RH_HIP_CHECK(hipMemcpyAsync(s.h_red, s.red_out, 5 * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    RH_HIP_CHECK(hipMemcpyAsync(h, s.t_dev, sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+    if (dt_last) *dt_last = dt_fixed > 0.0 ? dt_fixed : s.h_red[4];
+    if (t_total) *t_total = dt_fixed > 0.0 ? dt_fixed * nsteps : h[0];
+    if (sums)
+        for (int i = 0; i < 4; i++) sums[i] = s.h_red[i];
+    return 0;
+}
+
+int rh_time_godunov(int head_idx, int num_octs, int ilevel, double dx, double gamma, double smallr, double smallc2,
+                    double courant_factor, const double g[3], int slope, int riemann, double dt, int nlaunch,
+                    float *ms_per_launch)
+{
+    State &s = st();
+    if (!s.inited || !s.uold) return efail(RH_ERR_STATE, "library not initialised / mesh not allocated");
+    if (nlaunch < 1 || !(dt > 0)) return efail(RH_ERR_ARG, "nlaunch >= 1 and dt > 0 required");
+    if (!level_is_whole_mesh(head_idx, num_octs)) return efail(RH_ERR_STATE, "single-level mesh required");
+    GodArgs a3;
+    God1Args a1;
+    const LevelBricks *lb = nullptr;
+    int r = s.ndim == 3 ? fill_args3(head_idx, num_octs, ilevel, dx, gamma, smallr, smallc2, courant_factor, g, slope,
+                                     riemann, nullptr, dt, a3, lb)
+                        : fill_args1(head_idx, num_octs, ilevel, dx, gamma, smallr, smallc2, courant_factor, g,
+                                     slope, riemann, nullptr, dt, a1);
+    if (r) return r;
+    RH_HIP_CHECK(hipEventRecord(s.ev0, s.stream));
+    for (int k = 0; k < nlaunch; k++) {
+        if (s.ndim == 3) {
+            a3.uold = s.uold;
+            a3.unew = s.unew;
+            a3.ubase = s.uold;
+            RH_HIP_CHECK(launch_godunov3d(a3, lb->bx, lb->by, lb->bz, s.precision, s.stream));
+        } else {
+            a1.uold = s.uold;
+            a1.unew = s.unew;
+            a1.ubase = s.uold;
+            RH_HIP_CHECK(launch_godunov1d(a1, s.precision, s.stream));
+        }
+        std::swap(s.uold, s.unew);
+    }
+    RH_HIP_CHECK(hipEventRecord(s.ev1, s.stream));
+    RH_HIP_CHECK(hipEventSynchronize(s.ev1));
+    float t = 0;
+    RH_HIP_CHECK(hipEventElapsedTime(&t, s.ev0, s.ev1));
+    if (ms_per_launch) *ms_per_launch = t / nlaunch;
+    const size_t per = (size_t)s.nvar * s.ttd;
+    RH_HIP_CHECK(hipMemcpyAsync(s.unew + per * (head_idx - 1), s.uold + per * (head_idx - 1),
+                                per * num_octs * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+    RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+    return 0;
+}
+
+}  // extern "C"

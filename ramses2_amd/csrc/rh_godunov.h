@@ -1,0 +1,72 @@
+// Kernels of the fine-level Godunov step: oct-brick z-marching kernel (3D),
+// cell kernel (1D).  Shared by the drop-in entry points and by the batched /
+// serving extensions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rh_physics.h"
+
+namespace rh {
+
+// Arguments of one Godunov launch over the bricks of one level (or, for the
+// batched path, over a work list of bricks of several levels/jobs).
+struct GodArgs {
+    const double *uold;       // stencil source
+    double *unew;             // destination
+    const double *ubase;      // value the update is added to: unew (accumulate) or uold (fused)
+    const uint8_t *refined;   // [oct][8]
+    const int32_t *win;       // brick windows
+    const int32_t *org;       // brick origins (oct keys)
+    int nbricks;
+    int head, noct;           // 1-based active oct range of the level
+    int boxmin[3], boxmax[3];
+    int btype[3][2];
+    const double *dt_dev;     // if non-null, dt is read from device memory
+    double dt;
+    double dx, inv_dx;
+    double gamma, smallr, smallc2, courant_factor;
+    double smax_cap;          // used when use_smax != 0, else courant_factor*dx/dt
+    int use_smax;
+    double g[3];
+    int slope, riemann;
+    int xcd_remap;            // remap block ids so each XCD gets contiguous bricks
+};
+
+// 1D arguments: one thread per cell of the level, 3-slot neighbour table.
+struct God1Args {
+    const double *uold;
+    double *unew;
+    const double *ubase;
+    const uint8_t *refined;
+    const int32_t *nbor;      // [noct][3]
+    int head, noct;
+    int boxmin, boxmax;
+    int btype[2];
+    const double *dt_dev;
+    double dt, dx, inv_dx;
+    double gamma, smallr, smallc2, courant_factor, smax_cap;
+    int use_smax;
+    double g;
+    int slope, riemann;
+};
+
+// Launch the 3D brick kernel for brick shape (bx,by,bz) in precision prec
+// (0 = FP64, 1 = FP32 reconstruction/fluxes with FP64 accumulation).
+hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, hipStream_t s);
+hipError_t launch_godunov1d(const God1Args &a, int prec, hipStream_t s);
+// Persistent work-list launch: items[i] = (descriptor index, brick index).
+hipError_t launch_godunov3d_wl(const GodArgs *descs, const int2 *items, int nitems, int nblocks, int bx, int by,
+                               int bz, int prec, hipStream_t s);
+
+// Courant reduction over cells of octs [head, head+noct): per-block partials,
+// then a fixed-order final pass -> out[0..4] = mass, ekin, eint, emag, dt.
+hipError_t launch_courant(const double *uold, const uint8_t *refined, int head, int noct, int ndim,
+                          double dx, double gamma, double smallr, double smallc, double courant_factor,
+                          const double g[3], double *partials, int max_blocks, double *out,
+                          double *dt_out, hipStream_t s, double *t_acc = nullptr);
+
+// Ordered brick-shape support list: returns true when (bx,by,bz) has a compiled kernel.
+bool brick_shape_supported(int bx, int by, int bz);
+
+}  // namespace rh

@@ -1,0 +1,652 @@
+// gfx950 kernels of the fine-level Godunov hydro step.
+//
+// 3D: one workgroup per oct brick (BX*BY*BZ octs = 2BX x 2BY x 2BZ cells).
+// The workgroup marches through the brick along z, one cell plane per
+// iteration.  One thread owns one cell column of the (2BX+2) x (2BY+2) trace
+// region (the brick's cells plus one cell of halo).  Per plane:
+//   P: gather plane t of the (2BX+4) x (2BY+4) primitive region from uold
+//      through the brick window (hash-resolved oct indices, boundary ghosts
+//      synthesised on the fly), convert to primitives in FP64, keep a ring of
+//      three planes in LDS;
+//   T: slopes (uslope) and Hancock predictor (trace3d) of plane t-1, once per
+//      cell; left states of x/y faces go to LDS, z states stay in registers;
+//   F: each column solves the Riemann problem on its own low x, y and z face
+//      (cmpflxm), so every face is solved once; refined faces get zero flux;
+//   U: conservative update, accumulated as ((u + dFx) + dFy) + dFz in the
+//      order of godfine1 (hydro/godunov_fine.f90:737-816), written to unew.
+// No MFMA: the work is divides, square roots and compares, not a contraction.
+#include "rh_godunov.h"
+
+namespace rh {
+
+// Gather the conservative state of cell (i2,j2,k2) of the oct at window slot
+// value v, whose oct key is okey.  Boundary ghosts (v<0) follow
+// init_bound_refine (amr/boundaries.f90:39-165): zero gradient copies the
+// adjacent interior cell, reflexive mirrors it and flips the normal momentum.
+__device__ __forceinline__ void gather_cell(const double *__restrict__ uold, const uint8_t *__restrict__ refined,
+                                            int v, const int okey[3], int ndim, const int boxmin[3],
+                                            const int boxmax[3], const int btype[3][2], int c[3], double u[5],
+                                            uint8_t &rf)
+{
+    if (v > 0) {
+        const int ind = c[0] + 2 * c[1] + 4 * c[2];
+        const double *p = uold + (size_t)(v - 1) * 40 + ind;
+        if (ndim == 1) p = uold + (size_t)(v - 1) * 10 + ind;
+        const int stride = ndim == 1 ? 2 : 8;
+#pragma unroll
+        for (int n = 0; n < 5; n++) u[n] = p[n * stride];
+        rf = refined[(size_t)(v - 1) * 8 + ind];
+        return;
+    }
+    if (v == 0) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) u[n] = 0.0;
+        rf = 0;
+        return;
+    }
+    // boundary ghost: first direction in which the oct key leaves the domain
+    int dir = 0, side = 0;
+    for (int d = ndim - 1; d >= 0; d--) {
+        if (okey[d] < boxmin[d]) { dir = d; side = 0; }
+        else if (okey[d] >= boxmax[d]) { dir = d; side = 1; }
+    }
+    const int type = btype[dir][side];
+    int s[3] = {c[0], c[1], c[2]};
+    uint8_t rflag_src_valid;
+    int rs[3] = {c[0], c[1], c[2]};
+    if (side == 0) {
+        // reference oct lies at +1: refined(ind) = ref cell with c_dir-1 (ref_right)
+        rflag_src_valid = c[dir] == 1;
+        rs[dir] = 0;
+    } else {
+        rflag_src_valid = c[dir] == 0;
+        rs[dir] = 1;
+    }
+    if (type == 1)
+        s[dir] = 1 - c[dir];
+    else
+        s[dir] = side == 0 ? 0 : 1;
+    const int ref = -v;
+    const int ind = s[0] + 2 * s[1] + 4 * s[2];
+    const int stride = ndim == 1 ? 2 : 8;
+    const double *p = uold + (size_t)(ref - 1) * 5 * stride + ind;
+#pragma unroll
+    for (int n = 0; n < 5; n++) u[n] = p[n * stride];
+    if (type == 1) u[1 + dir] = -u[1 + dir];
+    rf = rflag_src_valid ? refined[(size_t)(ref - 1) * 8 + rs[0] + 2 * rs[1] + 4 * rs[2]] : 0;
+}
+
+template <typename T>
+__device__ __forceinline__ HydroK<T> make_k(const double dt, double dx, double gamma, double smallr, double smallc2,
+                                            double courant_factor, double smax_cap, int use_smax, int slope,
+                                            int riemann)
+{
+    HydroK<T> k;
+    k.gamma = T(gamma);
+    k.smallr = T(smallr);
+    k.smallc2 = T(smallc2);
+    k.smallp_trace = T(smallr * smallc2);
+    k.smallp_riem = T(smallc2 / gamma);
+    k.entho = T(1.0 / (gamma - 1.0));
+    k.dtdx = T(dt / dx);
+    k.smax_cap = T(use_smax ? smax_cap : courant_factor * dx / dt);
+    k.slope_type = slope;
+    k.riemann = riemann;
+    return k;
+}
+
+// Positivity-preserving 27-point slope (slope_type 3, umuscl.f90:2207-2270)
+// for variable n of the cell at ring index (s0, pc).
+template <typename T, int PX, int NP>
+__device__ __forceinline__ void slope27(const T (*sq)[5][NP], int sm, int s0, int sp, int n, int pc, T dq3[3])
+{
+    const T qc = sq[s0][n][pc];
+    T vmin = T(0), vmax = T(0);
+    bool first = true;
+    const int sl[3] = {sm, s0, sp};
+#pragma unroll
+    for (int kk = 0; kk < 3; kk++)
+#pragma unroll
+        for (int jj = -1; jj <= 1; jj++)
+#pragma unroll
+            for (int ii = -1; ii <= 1; ii++) {
+                T df = sq[sl[kk]][n][pc + ii + jj * PX] - qc;
+                if (first) {
+                    vmin = vmax = df;
+                    first = false;
+                } else {
+                    vmin = tmin(vmin, df);
+                    vmax = tmax(vmax, df);
+                }
+            }
+    T dfx = T(0.5) * (sq[s0][n][pc + 1] - sq[s0][n][pc - 1]);
+    T dfy = T(0.5) * (sq[s0][n][pc + PX] - sq[s0][n][pc - PX]);
+    T dfz = T(0.5) * (sq[sp][n][pc] - sq[sm][n][pc]);
+    T dff = T(0.5) * (fabs_(dfx) + fabs_(dfy) + fabs_(dfz));
+    T slop = dff > T(0) ? tmin(T(1), tdiv(tmin(fabs_(vmin), fabs_(vmax)), dff)) : T(1);
+    dq3[0] = slop * dfx;
+    dq3[1] = slop * dfy;
+    dq3[2] = slop * dfz;
+}
+
+// LDS of one brick workgroup.
+template <int BX, int BY, int BZ, typename T> struct BrickSmem {
+    static constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
+    static constexpr int PX = 2 * BX + 4, PY = 2 * BY + 4, NP = PX * PY;
+    static constexpr int WX = BX + 2, WY = BY + 2, WZ = BZ + 2, NW = WX * WY * WZ;
+    int32_t swin[NW];
+    T sq[3][5][NP];
+    uint8_t sref[3][NP];
+    T sf[2][5][NT];
+};
+
+// One brick: the body shared by the per-level kernel and the work-list kernel.
+template <int BX, int BY, int BZ, typename T>
+__device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, BrickSmem<BX, BY, BZ, T> &sm)
+{
+    constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
+    constexpr int PX = 2 * BX + 4, PY = 2 * BY + 4, NP = PX * PY;
+    constexpr int WX = BX + 2, WY = BY + 2, WZ = BZ + 2, NW = WX * WY * WZ;
+    int32_t *swin = sm.swin;
+    auto &sq = sm.sq;
+    auto &sref = sm.sref;
+    auto &sf = sm.sf;
+    (void)CY;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NW; i += NT) swin[i] = a.win[(size_t)bid * NW + i];
+    const int ox0 = a.org[3 * bid + 0], oy0 = a.org[3 * bid + 1], oz0 = a.org[3 * bid + 2];
+
+    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    const HydroK<T> k = make_k<T>(dt, a.dx, a.gamma, a.smallr, a.smallc2, a.courant_factor, a.smax_cap, a.use_smax,
+                                  a.slope, a.riemann);
+    const double smalle = a.smallc2 / a.gamma / (a.gamma - 1.0);
+    const double dtxhalf = dt * 0.5;
+    const double gdt[3] = {a.g[0] * dtxhalf, a.g[1] * dtxhalf, a.g[2] * dtxhalf};
+    const double inv_dx = a.inv_dx;
+    const int boxmin[3] = {a.boxmin[0], a.boxmin[1], a.boxmin[2]};
+    const int boxmax[3] = {a.boxmax[0], a.boxmax[1], a.boxmax[2]};
+    __syncthreads();
+
+    const int cx = tid % CX - 1, cy = tid / CX - 1;
+    const int pc = (cy + 2) * PX + (cx + 2);
+    const bool interior = cx >= 0 && cx < 2 * BX && cy >= 0 && cy < 2 * BY;
+    const bool do_fx = cx >= 0 && cx <= 2 * BX && cy >= 0 && cy < 2 * BY;
+    const bool do_fy = cy >= 0 && cy <= 2 * BY && cx >= 0 && cx < 2 * BX;
+
+    T qmz_prev[5] = {T(0), T(0), T(0), T(0), T(0)};
+    uint8_t ref_prev = 0;
+    double fz_prev[5] = {0, 0, 0, 0, 0}, part_prev[5] = {0, 0, 0, 0, 0};
+
+    for (int t = -2; t <= 2 * BZ + 1; ++t) {
+        const int st = (t + 3) % 3;
+        // ---- P: primitives of plane t ----
+        for (int i = tid; i < NP; i += NT) {
+            const int px = i % PX - 2, py = i / PX - 2;
+            const int ox = (px + 2) >> 1, oy = (py + 2) >> 1, oz = (t + 2) >> 1;
+            const int v = swin[(oz * WY + oy) * WX + ox];
+            int okey[3] = {ox0 + ox - 1, oy0 + oy - 1, oz0 + oz - 1};
+            int c[3] = {px & 1, py & 1, t & 1};
+            double u[5], q[5];
+            uint8_t rf;
+            gather_cell(a.uold, a.refined, v, okey, 3, boxmin, boxmax, a.btype, c, u, rf);
+            prim_from_cons(u, q, a.smallr, smalle, a.gamma, gdt);
+#pragma unroll
+            for (int n = 0; n < 5; n++) sq[st][n][i] = T(q[n]);
+            sref[st][i] = rf;
+        }
+        __syncthreads();
+        if (t < 0) continue;
+        // ---- T: slopes and predictor of plane tt = t-1 ----
+        const int tt = t - 1;
+        const int sm = (t + 1) % 3, s0 = (t + 2) % 3, sp = st;
+        T q0[5], dq[3][5];
+#pragma unroll
+        for (int n = 0; n < 5; n++) q0[n] = sq[s0][n][pc];
+        if (k.slope_type == 3) {
+#pragma unroll
+            for (int n = 0; n < 5; n++) {
+                T d3[3];
+                slope27<T, PX, NP>(sq, sm, s0, sp, n, pc, d3);
+                dq[0][n] = d3[0];
+                dq[1][n] = d3[1];
+                dq[2][n] = d3[2];
+            }
+        } else {
+            const int sl = k.slope_type;
+#pragma unroll
+            for (int n = 0; n < 5; n++) {
+                dq[0][n] = sl ? slope1(sl, sq[s0][n][pc - 1], q0[n], sq[s0][n][pc + 1]) : T(0);
+                dq[1][n] = sl ? slope1(sl, sq[s0][n][pc - PX], q0[n], sq[s0][n][pc + PX]) : T(0);
+                dq[2][n] = sl ? slope1(sl, sq[sm][n][pc], q0[n], sq[sp][n][pc]) : T(0);
+            }
+        }
+        if (k.slope_type > 0) {
+            dq[0][1] = vel_bound(dq[0][1], sq[s0][0][pc - 1], sq[s0][1][pc - 1], q0[0], q0[1], sq[s0][0][pc + 1],
+                                 sq[s0][1][pc + 1]);
+            dq[1][2] = vel_bound(dq[1][2], sq[s0][0][pc - PX], sq[s0][2][pc - PX], q0[0], q0[2],
+                                 sq[s0][0][pc + PX], sq[s0][2][pc + PX]);
+            dq[2][3] = vel_bound(dq[2][3], sq[sm][0][pc], sq[sm][3][pc], q0[0], q0[3], sq[sp][0][pc], sq[sp][3][pc]);
+        }
+        T qp[3][5], qm[3][5];
+        trace3(q0, dq, qp, qm, k);
+        const uint8_t myref = sref[s0][pc];
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+            sf[0][n][tid] = qm[0][n];
+            sf[1][n][tid] = qm[1][n];
+        }
+        __syncthreads();
+        // ---- F: Riemann problems on the low x, y, z faces of this column ----
+        const bool plane_out = tt >= 0 && tt < 2 * BZ;
+        T fx[5], fy[5];
+        double fz[5] = {0, 0, 0, 0, 0};
+        const bool wfx = do_fx && plane_out, wfy = do_fy && plane_out;
+        if (wfx) {
+            T ql[5];
+#pragma unroll
+            for (int n = 0; n < 5; n++) ql[n] = sf[0][n][tid - 1];
+            face_flux(0, ql, qp[0], fx, k);
+            if (myref || sref[s0][pc - 1])
+#pragma unroll
+                for (int n = 0; n < 5; n++) fx[n] = T(0);
+        }
+        if (wfy) {
+            T ql[5];
+#pragma unroll
+            for (int n = 0; n < 5; n++) ql[n] = sf[1][n][tid - CX];
+            face_flux(1, ql, qp[1], fy, k);
+            if (myref || sref[s0][pc - PX])
+#pragma unroll
+                for (int n = 0; n < 5; n++) fy[n] = T(0);
+        }
+        if (interior && tt >= 0 && tt <= 2 * BZ) {
+            T f[5];
+            face_flux(2, qmz_prev, qp[2], f, k);
+            const bool z = myref || ref_prev;
+#pragma unroll
+            for (int n = 0; n < 5; n++) fz[n] = z ? 0.0 : (double(f[n]) * dt) * inv_dx;
+        }
+        __syncthreads();
+        if (wfx)
+#pragma unroll
+            for (int n = 0; n < 5; n++) sf[0][n][tid] = fx[n];
+        if (wfy)
+#pragma unroll
+            for (int n = 0; n < 5; n++) sf[1][n][tid] = fy[n];
+        __syncthreads();
+        // ---- U: conservative update ----
+        if (interior) {
+            double part[5] = {0, 0, 0, 0, 0};
+            if (plane_out) {
+                const int ox = (cx >> 1) + 1, oy = (cy >> 1) + 1, oz = (tt >> 1) + 1;
+                const int v = swin[(oz * WY + oy) * WX + ox];
+                const int ind = (cx & 1) + 2 * (cy & 1) + 4 * (tt & 1);
+                const double *pb = a.ubase + (size_t)(v > 0 ? v - 1 : 0) * 40 + ind;
+#pragma unroll
+                for (int n = 0; n < 5; n++) {
+                    const double fl = (double(sf[0][n][tid]) * dt) * inv_dx;
+                    const double fr = (double(sf[0][n][tid + 1]) * dt) * inv_dx;
+                    const double gl = (double(sf[1][n][tid]) * dt) * inv_dx;
+                    const double gr = (double(sf[1][n][tid + CX]) * dt) * inv_dx;
+                    part[n] = (pb[8 * n] + (fl - fr)) + (gl - gr);
+                }
+            }
+            const int tf = tt - 1;
+            if (tf >= 0 && tf < 2 * BZ) {
+                const int ox = (cx >> 1) + 1, oy = (cy >> 1) + 1, oz = (tf >> 1) + 1;
+                const int v = swin[(oz * WY + oy) * WX + ox];
+                const bool own = v >= a.head && v < a.head + a.noct && ox0 + ox - 1 < boxmax[0] &&
+                                 oy0 + oy - 1 < boxmax[1] && oz0 + oz - 1 < boxmax[2];
+                if (own) {
+                    const int ind = (cx & 1) + 2 * (cy & 1) + 4 * (tf & 1);
+                    double *po = a.unew + (size_t)(v - 1) * 40 + ind;
+#pragma unroll
+                    for (int n = 0; n < 5; n++) po[8 * n] = part_prev[n] + (fz_prev[n] - fz[n]);
+                }
+            }
+#pragma unroll
+            for (int n = 0; n < 5; n++) {
+                part_prev[n] = part[n];
+                fz_prev[n] = fz[n];
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < 5; n++) qmz_prev[n] = qm[2][n];
+        ref_prev = myref;
+    }
+}
+
+template <int BX, int BY, int BZ, typename T>
+__global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2)) k_godunov3d(const GodArgs a)
+{
+    __shared__ BrickSmem<BX, BY, BZ, T> sm;
+    // XCD-aware brick assignment: consecutive bricks to the same XCD so that
+    // their shared halo octs meet in that XCD's L2.
+    int bid = blockIdx.x;
+    if (a.xcd_remap) {
+        const int nb = gridDim.x, x = bid & 7, l = bid >> 3;
+        const int q = nb >> 3, r = nb & 7;
+        bid = x * q + (x < r ? x : r) + l;
+    }
+    if (bid >= a.nbricks) return;
+    brick_body<BX, BY, BZ, T>(a, bid, sm);
+}
+
+// Persistent work-list kernel: work item i = (descriptor, brick).  Descriptors
+// carry the arrays and parameters of one level of one job, so bricks of
+// several levels and of several jobs share one launch.
+template <int BX, int BY, int BZ, typename T>
+__global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2))
+    k_godunov3d_wl(const GodArgs *__restrict__ descs, const int2 *__restrict__ items, int nitems)
+{
+    __shared__ BrickSmem<BX, BY, BZ, T> sm;
+    for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
+        const int2 it = items[i];
+        brick_body<BX, BY, BZ, T>(descs[it.x], it.y, sm);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 1D: one thread per cell; stencil of 5 cells from the 3-slot neighbour table.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_godunov1d(const God1Args a)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= 2 * a.noct) return;
+    const int o = a.head + (gid >> 1);  // 1-based oct
+    const int i2 = gid & 1;
+    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    const HydroK<T> k = make_k<T>(dt, a.dx, a.gamma, a.smallr, a.smallc2, a.courant_factor, a.smax_cap, a.use_smax,
+                                  a.slope < 2 ? a.slope : 2, a.riemann);
+    const double smalle = a.smallc2 / a.gamma / (a.gamma - 1.0);
+    const double gdt[3] = {a.g * (dt * 0.5), 0.0, 0.0};
+    const int32_t *nb = a.nbor + (size_t)(o - a.head) * 3;
+    const int boxmin[3] = {a.boxmin, 0, 0}, boxmax[3] = {a.boxmax, 1, 1};
+    const int btype[3][2] = {{a.btype[0], a.btype[1]}, {2, 2}, {2, 2}};
+    T q[5][5];
+    uint8_t rf[5];
+    for (int d = -2; d <= 2; d++) {
+        const int p = i2 + d;
+        const int slot = (p + 2) >> 1;
+        const int v = nb[slot];
+        // key of that oct relative to the domain: self oct key is unknown here,
+        // the slot tells the side (slot 0 = low, 2 = high)
+        int okey[3] = {slot == 0 ? a.boxmin - 1 : (slot == 2 ? a.boxmax : a.boxmin), 0, 0};
+        int c[3] = {p & 1, 0, 0};
+        double u[5], qq[5];
+        gather_cell(a.uold, a.refined, v, okey, 1, boxmin, boxmax, btype, c, u, rf[d + 2]);
+        prim_from_cons(u, qq, a.smallr, smalle, a.gamma, gdt);
+#pragma unroll
+        for (int n = 0; n < 5; n++) q[d + 2][n] = T(qq[n]);
+    }
+    T qp[3][5], qm[3][5];
+#pragma unroll
+    for (int c = 1; c <= 3; c++) {
+        T dq[5];
+#pragma unroll
+        for (int n = 0; n < 5; n++) dq[n] = k.slope_type ? slope1(k.slope_type, q[c - 1][n], q[c][n], q[c + 1][n]) : T(0);
+        if (k.slope_type > 0) dq[1] = vel_bound(dq[1], q[c - 1][0], q[c - 1][1], q[c][0], q[c][1], q[c + 1][0], q[c + 1][1]);
+        trace1(q[c], dq, qp[c - 1], qm[c - 1], k);
+    }
+    T fl[5], fr[5];
+    face_flux(0, qm[0], qp[1], fl, k);
+    face_flux(0, qm[1], qp[2], fr, k);
+    const bool zl = rf[1] || rf[2], zr = rf[2] || rf[3];
+    const int ind = i2;
+    const double *pb = a.ubase + (size_t)(o - 1) * 10 + ind;
+    double *po = a.unew + (size_t)(o - 1) * 10 + ind;
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+        const double FL = zl ? 0.0 : (double(fl[n]) * dt) * a.inv_dx;
+        const double FR = zr ? 0.0 : (double(fr[n]) * dt) * a.inv_dx;
+        po[2 * n] = pb[2 * n] + (FL - FR);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Courant time step and conservation sums (courant_fine + cmpdt).
+// ---------------------------------------------------------------------------
+struct CourantK {
+    double dx, vol, gamma, smallr, smallc, courant_factor, smallp, gsum_dx;
+    int ndim;
+};
+
+// cmpdt, hydro/godunov_utils.f90:228-337, for one cell (no MHD).
+__device__ __forceinline__ double cell_dt(double uu0, double uu1, double uu2, double uu3, double uu4, const CourantK &c)
+{
+    double r = uu0 > c.smallr ? uu0 : c.smallr;
+    double v[3] = {uu1 / r, uu2 / r, uu3 / r};
+    double e = uu4;
+    e = e - 0.5 * r * (v[0] * v[0]);
+    e = e - 0.5 * r * (v[1] * v[1]);
+    e = e - 0.5 * r * (v[2] * v[2]);
+    double pe = (c.gamma - 1.0) * e, pf = r * c.smallp;
+    double p = pe > pf ? pe : pf;
+    double a2 = c.gamma * p;
+    a2 = a2 / r;
+    double c2 = 0.5 * (0.0 / r + a2);
+    double cf = __builtin_sqrt(c2 + __builtin_sqrt(c2 * c2 - 0.0));
+    double ctot = 0.0;
+    for (int d = 0; d < c.ndim; d++) ctot = ctot + __builtin_fabs(v[d]) + cf;
+    double grav = c.gsum_dx / (ctot * ctot);
+    grav = grav > 0.0001 ? grav : 0.0001;
+    double dtmax = c.courant_factor * c.dx / c.smallc;
+    double dtcell = 2.0 * c.courant_factor * c.dx / (ctot * (1.0 + __builtin_sqrt(1.0 + 2.0 * c.courant_factor * grav)));
+    return dtmax < dtcell ? dtmax : dtcell;
+}
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+__device__ __forceinline__ double wave_min(double x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double y = __shfl_xor(x, o, 64);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
+constexpr int kCourantThreads = 256;
+
+__global__ void __launch_bounds__(kCourantThreads) k_courant(const double *__restrict__ uold,
+                                                             const uint8_t *__restrict__ refined, int head, int noct,
+                                                             int ttd, const CourantK c, double *__restrict__ part)
+{
+    double mass = 0, ekin = 0, eint = 0, dtmin = c.dx / c.smallc;
+    const long ncell = (long)noct * ttd;
+    const int stride = ttd == 8 ? 40 : 10;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += (long)gridDim.x * blockDim.x) {
+        const long o = (long)(head - 1) + i / ttd;
+        const int ind = (int)(i % ttd);
+        if (refined[o * 8 + ind]) continue;
+        const double *p = uold + o * stride + ind;
+        const double u0 = p[0], u1 = p[ttd], u2 = p[2 * ttd], u3 = p[3 * ttd], u4 = p[4 * ttd];
+        mass = mass + u0 * c.vol;
+        ekin = ekin + u4 * c.vol;
+        eint = eint + u4 * c.vol;
+        const double rr = u0 > c.smallr ? u0 : c.smallr;
+        eint = eint - 0.5 * (u1 * u1) / rr * c.vol;
+        eint = eint - 0.5 * (u2 * u2) / rr * c.vol;
+        eint = eint - 0.5 * (u3 * u3) / rr * c.vol;
+        const double d = cell_dt(u0, u1, u2, u3, u4, c);
+        dtmin = d < dtmin ? d : dtmin;
+    }
+    __shared__ double red[kCourantThreads / 64][4];
+    mass = wave_sum(mass);
+    ekin = wave_sum(ekin);
+    eint = wave_sum(eint);
+    dtmin = wave_min(dtmin);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) {
+        red[w][0] = mass;
+        red[w][1] = ekin;
+        red[w][2] = eint;
+        red[w][3] = dtmin;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0, s1 = 0, s2 = 0, m = red[0][3];
+        for (int i = 0; i < kCourantThreads / 64; i++) {
+            s0 += red[i][0];
+            s1 += red[i][1];
+            s2 += red[i][2];
+            m = red[i][3] < m ? red[i][3] : m;
+        }
+        part[4 * blockIdx.x + 0] = s0;
+        part[4 * blockIdx.x + 1] = s1;
+        part[4 * blockIdx.x + 2] = s2;
+        part[4 * blockIdx.x + 3] = m;
+    }
+}
+
+// Fixed-order reduction of the per-block partials: out = {mass, ekin, eint,
+// emag, dt}; dt_out (optional) receives dt for the next Godunov launch.
+__global__ void __launch_bounds__(256) k_courant_final(const double *__restrict__ part, int nblocks, double dtmax,
+                                                       double *__restrict__ out, double *__restrict__ dt_out,
+                                                       double *__restrict__ t_acc)
+{
+    __shared__ double red[4][4];
+    double s0 = 0, s1 = 0, s2 = 0, m = dtmax;
+    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
+        s0 += part[4 * i + 0];
+        s1 += part[4 * i + 1];
+        s2 += part[4 * i + 2];
+        m = part[4 * i + 3] < m ? part[4 * i + 3] : m;
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    m = wave_min(m);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) {
+        red[w][0] = s0;
+        red[w][1] = s1;
+        red[w][2] = s2;
+        red[w][3] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a0 = 0, a1 = 0, a2 = 0, mm = red[0][3];
+        for (int i = 0; i < 4; i++) {
+            a0 += red[i][0];
+            a1 += red[i][1];
+            a2 += red[i][2];
+            mm = red[i][3] < mm ? red[i][3] : mm;
+        }
+        out[0] = a0;
+        out[1] = a1;
+        out[2] = a2;
+        out[3] = 0.0;
+        out[4] = mm;
+        if (dt_out) *dt_out = mm;
+        if (t_acc) *t_acc += mm;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+#define RH_BRICK_SHAPES(X) \
+    X(7, 7, 8)             \
+    X(7, 7, 4)             \
+    X(3, 3, 4)             \
+    X(3, 3, 8)             \
+    X(3, 7, 8)             \
+    X(1, 1, 1)             \
+    X(1, 1, 2)
+
+bool brick_shape_supported(int bx, int by, int bz)
+{
+#define RH_CHECK_SHAPE(A, B, C) \
+    if (bx == A && by == B && bz == C) return true;
+    RH_BRICK_SHAPES(RH_CHECK_SHAPE)
+#undef RH_CHECK_SHAPE
+    return false;
+}
+
+hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, hipStream_t s)
+{
+    if (a.nbricks <= 0) return hipSuccess;
+#define RH_LAUNCH_SHAPE(A, B, C)                                                                      \
+    if (bx == A && by == B && bz == C) {                                                              \
+        constexpr int nt = (2 * A + 2) * (2 * B + 2);                                                  \
+        if (prec == 0)                                                                                \
+            hipLaunchKernelGGL((k_godunov3d<A, B, C, double>), dim3(a.nbricks), dim3(nt), 0, s, a);   \
+        else                                                                                          \
+            hipLaunchKernelGGL((k_godunov3d<A, B, C, float>), dim3(a.nbricks), dim3(nt), 0, s, a);    \
+        return hipGetLastError();                                                                     \
+    }
+    RH_BRICK_SHAPES(RH_LAUNCH_SHAPE)
+#undef RH_LAUNCH_SHAPE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_godunov3d_wl(const GodArgs *descs, const int2 *items, int nitems, int nblocks, int bx, int by,
+                               int bz, int prec, hipStream_t s)
+{
+    if (nitems <= 0) return hipSuccess;
+#define RH_LAUNCH_WL(A, B, C)                                                                                   \
+    if (bx == A && by == B && bz == C) {                                                                        \
+        constexpr int nt = (2 * A + 2) * (2 * B + 2);                                                            \
+        if (prec == 0)                                                                                          \
+            hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, double>), dim3(nblocks), dim3(nt), 0, s, descs, items,  \
+                               nitems);                                                                         \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, float>), dim3(nblocks), dim3(nt), 0, s, descs, items,   \
+                               nitems);                                                                         \
+        return hipGetLastError();                                                                               \
+    }
+    RH_BRICK_SHAPES(RH_LAUNCH_WL)
+#undef RH_LAUNCH_WL
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_godunov1d(const God1Args &a, int prec, hipStream_t s)
+{
+    const int n = 2 * a.noct;
+    if (n <= 0) return hipSuccess;
+    const int nb = (n + 255) / 256;
+    if (prec == 0)
+        hipLaunchKernelGGL((k_godunov1d<double>), dim3(nb), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_godunov1d<float>), dim3(nb), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_courant(const double *uold, const uint8_t *refined, int head, int noct, int ndim, double dx,
+                          double gamma, double smallr, double smallc, double courant_factor, const double g[3],
+                          double *partials, int max_blocks, double *out, double *dt_out, hipStream_t s,
+                          double *t_acc)
+{
+    CourantK c;
+    c.dx = dx;
+    c.vol = ndim == 1 ? dx : dx * dx * dx;
+    c.gamma = gamma;
+    c.smallr = smallr;
+    c.smallc = smallc;
+    c.courant_factor = courant_factor;
+    c.smallp = (smallc * smallc) / gamma;
+    double gs = 0.0;
+    for (int d = 0; d < ndim; d++) gs = gs + __builtin_fabs(g[d]);
+    c.gsum_dx = gs * dx;
+    c.ndim = ndim;
+    const int ttd = 1 << ndim;
+    const long ncell = (long)noct * ttd;
+    int nb = (int)((ncell + kCourantThreads - 1) / kCourantThreads);
+    if (nb > max_blocks) nb = max_blocks;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_courant, dim3(nb), dim3(kCourantThreads), 0, s, uold, refined, head, noct, ttd, c, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(256), 0, s, partials, nb, dx / smallc, out, dt_out, t_acc);
+    return hipGetLastError();
+}
+
+}  // namespace rh

@@ -1,0 +1,307 @@
+// Device physics of the MUSCL-Hancock Godunov step, templated on the
+// reconstruction/flux precision T (double for the FP64 kernel, float for the
+// mixed kernel).  Conversion to primitive variables (the pressure recovery
+// p=(gamma-1)(E-rho v^2/2)) is always done in double; see rh_prim_from_cons.
+//
+// Each routine restates the reference CPU routine it names, in the same
+// expression order, so that the FP64 instantiation agrees with the reference
+// to rounding (FMA contraction aside).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rh {
+
+// Scalar helpers -----------------------------------------------------------------
+template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+__device__ __forceinline__ double tcopysign(double a, double b) { return __builtin_copysign(a, b); }
+__device__ __forceinline__ float tcopysign(float a, float b) { return __builtin_copysignf(a, b); }
+__device__ __forceinline__ double fabs_(double a) { return __builtin_fabs(a); }
+__device__ __forceinline__ float fabs_(float a) { return __builtin_fabsf(a); }
+
+// Division and square root: IEEE in double; the hardware reciprocal and square
+// root (about 1 ulp) in float, where the mixed kernel tolerates 1e-7 relative.
+__device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float tdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double tsqrt(double a) { return __builtin_sqrt(a); }
+__device__ __forceinline__ float tsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+
+// Constants of one Godunov launch (hydro_params_t + derived quantities).
+template <typename T> struct HydroK {
+    T gamma;
+    T smallr;
+    T smallc2;
+    T smallp_trace;   // smallr*smallc**2       (trace3d, umuscl.f90:934)
+    T smallp_riem;    // smallc**2/gamma        (riemann_*, godunov_utils.f90:392)
+    T entho;          // one/(gamma-one)
+    T dtdx;           // dt/dx                  (trace3d, umuscl.f90:929)
+    T smax_cap;       // courant_factor*dx/dt   (umuscl.f90:99)
+    int slope_type;
+    int riemann;
+};
+
+// ctoprim, hydro/umuscl.f90:1739-1901, for one cell, in double.
+// u: conservative (rho, rho u, rho v, rho w, E); q: (rho, u, v, w, p).
+__device__ __forceinline__ void prim_from_cons(const double u[5], double q[5], double smallr, double smalle,
+                                               double gamma, const double gdt_half[3])
+{
+    double r = u[0] > smallr ? u[0] : smallr;
+    double oneoverrho = 1.0 / r;
+    double vx = u[1] * oneoverrho;
+    double vy = u[2] * oneoverrho;
+    double vz = u[3] * oneoverrho;
+    double ekin = 0.5 * r * (vx * vx);
+    ekin = ekin + 0.5 * r * (vy * vy);
+    ekin = ekin + 0.5 * r * (vz * vz);
+    double e = u[4] - ekin;
+    double se = smalle * r;
+    double eint = e > se ? e : se;
+    q[0] = r;
+    q[1] = vx + gdt_half[0];
+    q[2] = vy + gdt_half[1];
+    q[3] = vz + gdt_half[2];
+    q[4] = (gamma - 1.0) * eint;
+}
+
+// uslope, hydro/umuscl.f90:1906-2387: one-dimensional limited slope.
+// slope_type 1 = MinMod (:2132-2170), 2 = MonCen (:2171-2206).
+template <typename T> __device__ __forceinline__ T slope1(int st, T qm1, T q0, T qp1)
+{
+    if (st == 1) {
+        T dlft = q0 - qm1;
+        T drgt = qp1 - q0;
+        if ((dlft * drgt) <= T(0)) return T(0);
+        return dlft > T(0) ? tmin(dlft, drgt) : tmax(dlft, drgt);
+    }
+    T dlft = T(2) * (q0 - qm1);
+    T drgt = T(2) * (qp1 - q0);
+    T dcen = T(0.5) * (dlft + drgt) * T(0.5);
+    T dsgn = tcopysign(T(1), dcen);
+    T slop = tmin(fabs_(dlft), fabs_(drgt));
+    T dlim = ((dlft * drgt) <= T(0)) ? T(0) : slop;
+    return dsgn * tmin(dlim, fabs_(dcen));
+}
+
+// Normal-velocity slope bound, hydro/umuscl.f90:2361-2385.
+template <typename T>
+__device__ __forceinline__ T vel_bound(T dq, T rl, T ul, T r0, T u0, T rr, T ur)
+{
+    T dlft = rl * fabs_(u0 - ul);
+    T drgt = rr * fabs_(ur - u0);
+    T dlim = tdiv(T(2) * tmin(dlft, drgt), r0);
+    return tcopysign(tmin(fabs_(dq), dlim), dq);
+}
+
+// trace3d, hydro/umuscl.f90:860-1466, hydro part for one cell.
+// q: cell primitives; dq[d][n]: slopes.  Outputs qp[d][n] (right state at the
+// left interface) and qm[d][n] (left state at the right interface).
+template <typename T>
+__device__ __forceinline__ void trace3(const T q[5], const T dq[3][5], T qp[3][5], T qm[3][5], const HydroK<T> &k)
+{
+    const T half = T(0.5);
+    T r = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
+    T drx = half * dq[0][0], dux = half * dq[0][1], dvx = half * dq[0][2], dwx = half * dq[0][3], dpx = half * dq[0][4];
+    T dry = half * dq[1][0], duy = half * dq[1][1], dvy = half * dq[1][2], dwy = half * dq[1][3], dpy = half * dq[1][4];
+    T drz = half * dq[2][0], duz = half * dq[2][1], dvz = half * dq[2][2], dwz = half * dq[2][3], dpz = half * dq[2][4];
+    T divv = dux + dvy + dwz;
+    T sr0 = -u * drx - v * dry - w * drz - divv * r;
+    T su0 = -u * dux - v * duy - w * duz - tdiv(dpx, r);
+    T sv0 = -u * dvx - v * dvy - w * dvz - tdiv(dpy, r);
+    T sw0 = -u * dwx - v * dwy - w * dwz - tdiv(dpz, r);
+    T sp0 = -u * dpx - v * dpy - w * dpz - divv * k.gamma * p;
+    T ws[5];
+    ws[0] = r + sr0 * k.dtdx;
+    ws[1] = u + su0 * k.dtdx;
+    ws[2] = v + sv0 * k.dtdx;
+    ws[3] = w + sw0 * k.dtdx;
+    ws[4] = p + sp0 * k.dtdx;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+            T h = half * dq[d][n];
+            qp[d][n] = ws[n] - h;
+            qm[d][n] = ws[n] + h;
+        }
+        if (qp[d][0] < k.smallr) qp[d][0] = r;
+        if (qp[d][4] < k.smallp_trace) qp[d][4] = p;
+        if (qm[d][0] < k.smallr) qm[d][0] = r;
+        if (qm[d][4] < k.smallp_trace) qm[d][4] = p;
+    }
+}
+
+// trace1d, hydro/umuscl.f90:297-481.
+template <typename T>
+__device__ __forceinline__ void trace1(const T q[5], const T dq[5], T qp[5], T qm[5], const HydroK<T> &k)
+{
+    const T half = T(0.5);
+    T r = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
+    T drx = half * dq[0], dux = half * dq[1], dvx = half * dq[2], dwx = half * dq[3], dpx = half * dq[4];
+    T sr0 = -u * drx - dux * r;
+    T sp0 = -u * dpx - dux * k.gamma * p;
+    T su0 = -u * dux - tdiv(dpx, r);
+    T sv0 = -u * dvx;
+    T sw0 = -u * dwx;
+    T ws[5];
+    ws[0] = r + sr0 * k.dtdx;
+    ws[1] = u + su0 * k.dtdx;
+    ws[2] = v + sv0 * k.dtdx;
+    ws[3] = w + sw0 * k.dtdx;
+    ws[4] = p + sp0 * k.dtdx;
+    T h[5] = {drx, dux, dvx, dwx, dpx};
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+        qp[n] = ws[n] - h[n];
+        qm[n] = ws[n] + h[n];
+    }
+    if (qp[0] < k.smallr) qp[0] = r;
+    if (qp[4] < k.smallp_trace) qp[4] = p;
+    if (qm[0] < k.smallr) qm[0] = r;
+    if (qm[4] < k.smallp_trace) qm[4] = p;
+}
+
+// riemann_hllc, hydro/godunov_utils.f90:638-857 (mass, momenta, energy fluxes).
+// Only the star state of the side that is sampled is formed; the sampled
+// values are those of the reference, which forms both.
+template <typename T>
+__device__ __forceinline__ void riemann_hllc(const T ql[5], const T qr[5], T f[5], const HydroK<T> &k)
+{
+    const T half = T(0.5);
+    T rl = tmax(ql[0], k.smallr);
+    T ul = ql[1], vl = ql[2], wl = ql[3];
+    T Pl = tmax(ql[4], rl * k.smallp_riem);
+    T el = Pl * k.entho;
+    T etotl = el + half * rl * (ul * ul + vl * vl + wl * wl);
+    T rr = tmax(qr[0], k.smallr);
+    T ur = qr[1], vr = qr[2], wr = qr[3];
+    T Pr = tmax(qr[4], rr * k.smallp_riem);
+    T er = Pr * k.entho;
+    T etotr = er + half * rr * (ur * ur + vr * vr + wr * wr);
+    T cfastl = tsqrt(tmax(tdiv(k.gamma * Pl, rl), k.smallc2));
+    T cfastr = tsqrt(tmax(tdiv(k.gamma * Pr, rr), k.smallc2));
+    T cmax = tmax(cfastl, cfastr);
+    T SL = tmin(ul, ur) - cmax;
+    T SR = tmax(ul, ur) + cmax;
+    T rcl = rl * (ul - SL);
+    T rcr = rr * (SR - ur);
+    T den = rcr + rcl;
+    T ustar = tdiv(rcr * ur + rcl * ul + (Pl - Pr), den);
+    T Ptotstar = tdiv(rcr * Pl + rcl * Pr + rcl * rcr * (ul - ur), den);
+    bool lstar = ustar > T(0);
+    T S = lstar ? SL : SR;
+    T rs = lstar ? rl : rr, us = lstar ? ul : ur, Ps = lstar ? Pl : Pr, es = lstar ? etotl : etotr;
+    T rstar = tdiv(rs * (S - us), S - ustar);
+    T etotstar = tdiv((S - us) * es - Ps * us + Ptotstar * ustar, S - ustar);
+    T ro, uo, vo, wo, Po, eo;
+    if (SL > T(0)) {
+        ro = rl; uo = ul; vo = vl; wo = wl; Po = Pl; eo = etotl;
+    } else if (lstar) {
+        ro = rstar; uo = ustar; vo = vl; wo = wl; Po = Ptotstar; eo = etotstar;
+    } else if (SR > T(0)) {
+        ro = rstar; uo = ustar; vo = vr; wo = wr; Po = Ptotstar; eo = etotstar;
+    } else {
+        ro = rr; uo = ur; vo = vr; wo = wr; Po = Pr; eo = etotr;
+    }
+    f[0] = ro * uo;
+    f[1] = ro * uo * uo + Po;
+    f[2] = ro * uo * vo;
+    f[3] = ro * uo * wo;
+    f[4] = (eo + Po) * uo;
+}
+
+// Conservative state and physical flux of one side (riemann_llf/hll,
+// godunov_utils.f90:420-470 and :565-615).
+template <typename T>
+__device__ __forceinline__ void side_state(const T q[5], T entho, T u[5])
+{
+    const T half = T(0.5);
+    u[0] = q[0];
+    u[1] = q[0] * q[1];
+    u[2] = q[0] * q[2];
+    u[3] = q[0] * q[3];
+    T e = q[4] * entho + half * q[0] * (q[1] * q[1]);
+    e = e + half * q[0] * (q[2] * q[2]);
+    u[4] = e + half * q[0] * (q[3] * q[3]);
+}
+
+// riemann_llf, hydro/godunov_utils.f90:338-485.
+template <typename T>
+__device__ __forceinline__ void riemann_llf(const T ql[5], const T qr[5], T f[5], const HydroK<T> &k)
+{
+    T rl = tmax(ql[0], k.smallr), pl = tmax(ql[4], rl * k.smallp_riem);
+    T cl = tsqrt(tdiv(k.gamma * pl, rl));
+    T rr = tmax(qr[0], k.smallr), pr = tmax(qr[4], rr * k.smallp_riem);
+    T cr = tsqrt(tdiv(k.gamma * pr, rr));
+    T cmax = tmin(tmax(fabs_(ql[1]) + cl, fabs_(qr[1]) + cr), k.smax_cap);
+    T uL[5], uR[5];
+    side_state(ql, k.entho, uL);
+    side_state(qr, k.entho, uR);
+    T fL[5], fR[5];
+    fL[0] = ql[1] * uL[0];
+    fR[0] = qr[1] * uR[0];
+    fL[1] = ql[1] * uL[1] + ql[4];
+    fR[1] = qr[1] * uR[1] + qr[4];
+    fL[2] = ql[1] * uL[2];
+    fR[2] = qr[1] * uR[2];
+    fL[3] = ql[1] * uL[3];
+    fR[3] = qr[1] * uR[3];
+    fL[4] = ql[1] * (uL[4] + ql[4]);
+    fR[4] = qr[1] * (uR[4] + qr[4]);
+#pragma unroll
+    for (int n = 0; n < 5; n++) f[n] = T(0.5) * (fL[n] + fR[n] - cmax * (uR[n] - uL[n]));
+}
+
+// riemann_hll, hydro/godunov_utils.f90:486-637.
+template <typename T>
+__device__ __forceinline__ void riemann_hll(const T ql[5], const T qr[5], T f[5], const HydroK<T> &k)
+{
+    T rl = tmax(ql[0], k.smallr), ul = ql[1], pl = tmax(ql[4], rl * k.smallp_riem);
+    T cl = tsqrt(tdiv(k.gamma * pl, rl));
+    T rr = tmax(qr[0], k.smallr), ur = qr[1], pr = tmax(qr[4], rr * k.smallp_riem);
+    T cr = tsqrt(tdiv(k.gamma * pr, rr));
+    T SL = tmax(tmin(tmin(ul, ur) - tmax(cl, cr), T(0)), -k.smax_cap);
+    T SR = tmin(tmax(tmax(ul, ur) + tmax(cl, cr), T(0)), k.smax_cap);
+    T uL[5], uR[5];
+    side_state(ql, k.entho, uL);
+    side_state(qr, k.entho, uR);
+    T fL[5], fR[5];
+    fL[0] = uL[1];
+    fR[0] = uR[1];
+    fL[1] = ql[4] + uL[1] * ql[1];
+    fR[1] = qr[4] + uR[1] * qr[1];
+    fL[2] = ql[1] * uL[2];
+    fR[2] = qr[1] * uR[2];
+    fL[3] = ql[1] * uL[3];
+    fR[3] = qr[1] * uR[3];
+    fL[4] = ql[1] * (uL[4] + ql[4]);
+    fR[4] = qr[1] * (uR[4] + qr[4]);
+#pragma unroll
+    for (int n = 0; n < 5; n++) f[n] = tdiv(SR * fL[n] - SL * fR[n] + SR * SL * (uR[n] - uL[n]), SR - SL);
+}
+
+// cmpflxm, hydro/umuscl.f90:1472-1607: rotate to the face normal idim, solve,
+// rotate back.  ql/qr are (rho,u,v,w,p) in grid axes.
+template <typename T>
+__device__ __forceinline__ void face_flux(int idim, const T ql[5], const T qr[5], T f[5], const HydroK<T> &k)
+{
+    const int ln = idim + 1;
+    const int lt1 = idim == 0 ? 2 : 1;
+    const int lt2 = idim == 2 ? 2 : 3;
+    T a[5] = {ql[0], ql[ln], ql[lt1], ql[lt2], ql[4]};
+    T b[5] = {qr[0], qr[ln], qr[lt1], qr[lt2], qr[4]};
+    T g[5];
+    if (k.riemann == 3)
+        riemann_hllc(a, b, g, k);
+    else if (k.riemann == 1)
+        riemann_llf(a, b, g, k);
+    else
+        riemann_hll(a, b, g, k);
+    f[0] = g[0];
+    f[ln] = g[1];
+    f[lt1] = g[2];
+    f[lt2] = g[3];
+    f[4] = g[4];
+}
+
+}  // namespace rh

@@ -1,0 +1,91 @@
+// Library state and shared declarations of the gfx950 hydro library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+namespace rh {
+
+// Status of the last call, reported through rh_last_error().
+void set_error(const std::string &msg);
+
+#define RH_HIP_CHECK(expr)                                                                  \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess) {                                                             \
+            ::rh::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));             \
+            return 1;                                                                       \
+        }                                                                                   \
+    } while (0)
+
+// Oct-brick decomposition of one level: bricks of BX*BY*BZ octs, each with a
+// window of (BX+2)*(BY+2)*(BZ+2) oct slots (the brick plus a one-oct halo).
+// Window values: >0 oct index (1-based), <0 boundary ghost built from oct
+// -value, 0 no oct.
+struct LevelBricks {
+    int ilevel = 0;
+    int head = 0, noct = 0;
+    int bx = 0, by = 0, bz = 0;
+    int nbricks = 0;
+    int32_t *win = nullptr;   // [nbricks][bz+2][by+2][bx+2]
+    int32_t *org = nullptr;   // [nbricks][3] oct key of the brick's first oct
+    int boxmin[3] = {0, 0, 0};
+    int boxmax[3] = {0, 0, 0};
+};
+
+// Neighbour table of one level: 3^ndim slots per oct of the level.
+struct LevelNbor {
+    int ilevel = 0;
+    int head = 0, noct = 0;
+    int32_t *nbor = nullptr;  // [noct][3^ndim]
+    int boxmin[3] = {0, 0, 0};
+    int boxmax[3] = {0, 0, 0};
+};
+
+struct State {
+    bool inited = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int ndim = 3, nvar = 5, ttd = 8;
+    int ngridmax = 0, ncachemax = 0, ngridtot = 0;
+    int64_t hash_cap = 0;   // power of two
+    double *uold = nullptr, *unew = nullptr;
+    int32_t *ckey = nullptr;       // [ngridtot][3]
+    int32_t *lev = nullptr;        // [ngridtot]
+    uint8_t *refined = nullptr;    // [ngridtot][8]
+    uint64_t *hkeys = nullptr;     // hash keys (empty = ~0)
+    int32_t *hvals = nullptr;
+    // per-level parameters (index l-1)
+    int nlevelmax = 0;
+    std::vector<int32_t> ckey_max;
+    std::vector<int64_t> key_off;
+    std::vector<int32_t> box_min, box_max;  // [l][3]
+    int periodic[3] = {1, 1, 1};
+    int btype[3][2] = {{2, 2}, {2, 2}, {2, 2}};
+    // host copy of the oct keys (brick enumeration)
+    std::vector<int32_t> h_ckey, h_lev;
+    std::map<int, LevelBricks> bricks;  // by level
+    std::map<int, LevelNbor> nbors;     // by level
+    // brick shape used for 3D levels
+    int brick_bx = 7, brick_by = 7, brick_bz = 8;
+    // Godunov precision: 0 FP64, 1 FP32 reconstruction/fluxes + FP64 accumulation
+    int precision = 0;
+    // timer events
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // single-level device time stepping: dt and accumulated time on the device
+    double *dt_dev = nullptr;
+    double *t_dev = nullptr;
+    // reduction scratch
+    double *red_part = nullptr;
+    int red_cap = 0;
+    double *red_out = nullptr;     // device, 8 doubles
+    double *h_red = nullptr;       // pinned host, 8 doubles
+    std::string last_error;
+};
+
+State &st();
+
+}  // namespace rh

@@ -1,0 +1,222 @@
+"""Host side of the drop-in boundary: the reference's level-kernel call sequence
+driven through the C-ABI of include/ramses2_hip.h.
+
+``HipHydro`` mirrors the Fortran runner (metal/metal_runner.f90): one method
+per rh_* entry point, same argument meaning, errors raised as exceptions (the
+runner's ``error stop``).  ``UniformLevel`` drives a single-level mesh through
+the reference's order of calls per time step (amr/amr_step.f90:276-455):
+newdt_fine -> courant_fine, set_unew, godunov_fine, set_uold.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, ptr
+from .mesh import NVAR, UniformMesh
+
+RIEMANN_IDS = {"llf": 1, "hll": 2, "hllc": 3}
+
+
+@dataclass
+class HydroParams:
+    """&HYDRO_PARAMS subset (amr/read_params.f90:318-354)."""
+
+    gamma: float = 1.4
+    courant_factor: float = 0.5
+    smallr: float = 1e-10
+    smallc: float = 1e-10
+    slope_type: int = 1
+    riemann: str = "llf"
+
+    @property
+    def riemann_id(self):
+        return RIEMANN_IDS[self.riemann] if isinstance(self.riemann, str) else int(self.riemann)
+
+
+class HipHydro:
+    """The device-resident hydro state of this process (one per process, as one
+    MPI rank owns one device in the reference)."""
+
+    def __init__(self, ndim, ngridmax, ncachemax=0, hash_size=None):
+        self.ndim = ndim
+        self.ttd = 2 ** ndim
+        self.ngridmax = ngridmax
+        self.ncachemax = ncachemax
+        self.hash_size = hash_size or 2 * (ngridmax + ncachemax)
+        call("rh_init")
+        call("rh_alloc_amr", ngridmax, ncachemax, NVAR, self.ttd, self.hash_size)
+        self.box = None
+
+    # ---- mesh set-up ----
+    def upload_level_params(self, ckey_max, key_off, box_min, box_max, periodic):
+        self.ckey_max = np.ascontiguousarray(ckey_max, dtype=np.int32)
+        self.key_off = np.ascontiguousarray(key_off, dtype=np.int64)
+        self.box_min = np.ascontiguousarray(box_min, dtype=np.int32)
+        self.box_max = np.ascontiguousarray(box_max, dtype=np.int32)
+        self.periodic = np.ascontiguousarray(periodic, dtype=np.int32)
+        nlevelmax = len(self.ckey_max) - 1
+        call("rh_upload_level_params", ptr(self.ckey_max, _lib._ip), ptr(self.key_off, _lib._lp),
+             ptr(self.box_min, _lib._ip), ptr(self.box_max, _lib._ip), ptr(self.periodic, _lib._ip), nlevelmax)
+
+    def set_boundaries(self, bound_type):
+        bt = np.ascontiguousarray(np.asarray(bound_type, dtype=np.int32).reshape(6))
+        call("rh_set_boundaries", ptr(bt, _lib._ip))
+
+    def set_grid_device(self, uold, unew, grid):
+        uold = np.ascontiguousarray(uold, dtype=np.float64)
+        unew = np.ascontiguousarray(unew, dtype=np.float64)
+        n = uold.shape[0]
+        assert uold.shape == (n, NVAR, self.ttd) and unew.shape == uold.shape and len(grid) == n
+        call("rh_set_grid_device", ptr(uold), ptr(unew), None, grid.ctypes.data_as(ctypes.c_void_p), n,
+             NVAR, self.ttd)
+        self.ngrid = n
+
+    def transfer_grid_host(self, n=None):
+        n = self.ngrid if n is None else n
+        u = np.zeros((n, NVAR, self.ttd))
+        call("rh_transfer_grid_host", ptr(u), None, n, NVAR, self.ttd)
+        return u
+
+    def transfer_unew_host(self, n=None):
+        n = self.ngrid if n is None else n
+        u = np.zeros((n, NVAR, self.ttd))
+        call("rh_transfer_unew_host", ptr(u), n, NVAR, self.ttd)
+        return u
+
+    def insert_hash_all(self, head, num):
+        call("rh_insert_hash_all", head, num, self.hash_size)
+
+    def build_nbor(self, head, num, ilevel):
+        l = ilevel - 1
+        bmin = np.ascontiguousarray(self.box_min[l])
+        bmax = np.ascontiguousarray(self.box_max[l])
+        call("rh_build_nbor", head, num, self.hash_size, int(self.ckey_max[l]), int(self.key_off[l]),
+             ptr(bmin, _lib._ip), ptr(bmax, _lib._ip), ptr(self.periodic, _lib._ip))
+
+    # ---- level kernels ----
+    def set_unew(self, head, num):
+        call("rh_set_unew", head, num)
+
+    def set_uold(self, head, num):
+        call("rh_set_uold", head, num)
+
+    def cmpdt(self, head, num, dx, p: HydroParams, gravity=(0.0, 0.0, 0.0)):
+        g = np.ascontiguousarray(gravity, dtype=np.float64)
+        out = [ctypes.c_double() for _ in range(5)]
+        call("rh_cmpdt", head, num, dx, p.gamma, p.smallr, p.smallc ** 2, p.courant_factor, ptr(g),
+             *[ctypes.byref(o) for o in out])
+        return dict(zip(("mass", "ekin", "eint", "emag", "dt"), [o.value for o in out]))
+
+    def godunov(self, head, num, ilevel, levelmin, levelmax, dt, dx, p: HydroParams, gravity=(0.0, 0.0, 0.0),
+                precision=0):
+        g = np.ascontiguousarray(gravity, dtype=np.float64)
+        call("rh_set_precision", int(precision))
+        smax_cap = p.courant_factor * dx / dt
+        call("rh_godunov", head, num, self.ngridmax, ilevel, levelmin, levelmax, p.gamma, p.smallr,
+             p.smallc ** 2, dt, dx, smax_cap, p.slope_type, p.riemann_id, ptr(g))
+
+    def sync(self):
+        call("rh_device_sync")
+
+    @staticmethod
+    def set_brick_shape(bx, by, bz):
+        call("rh_set_brick_shape", bx, by, bz)
+
+    @staticmethod
+    def brick_shape():
+        v = [ctypes.c_int32() for _ in range(3)]
+        call("rh_get_brick_shape", *[ctypes.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+
+def newdt(dx, p: HydroParams, courant_dt, t=None, tout=None, nlanding=0, dtold=0.0):
+    """dt rules of m_newdt_fine (pm/newdt_fine.f90:64-180) for one level without
+    gravity/particles: dtnew = min(dx/smallc, courant); then, within nlanding
+    steps of tout, land on it with dt = trem/nrem."""
+    dtnew = dx / p.smallc
+    dtnew = min(dtnew, courant_dt)
+    if nlanding > 0 and tout is not None and t is not None:
+        trem = tout - t
+        if trem > 0:
+            dtest = dtnew
+            if dtold > 0:
+                dtest = min(dtest, dtold)
+            if trem < nlanding * dtest:
+                nrem = max(1, int(np.ceil(trem / dtest - 1e-6)))
+                dtnew = trem / float(nrem)
+                if nrem == 1:
+                    dtnew = trem + 1e-10 * max(trem, abs(tout))
+    return dtnew
+
+
+class UniformLevel:
+    """A single uniform level (levelmin = levelmax) on the device."""
+
+    def __init__(self, mesh: UniformMesh, cons_cells, params: HydroParams, periodic=True, bound_type=2,
+                 ncachemax=0):
+        self.mesh = mesh
+        self.p = params
+        nd = mesh.ndim
+        self.hh = HipHydro(nd, mesh.noct, ncachemax)
+        ckey_max, key_off, box_min, box_max = mesh.level_params()
+        per = [1, 1, 1] if periodic else [0, 0, 0]
+        if nd == 1:
+            per = [int(periodic), 1, 1]
+        self.hh.upload_level_params(ckey_max, key_off, box_min, box_max, per)
+        self.hh.set_boundaries([bound_type] * 6)
+        u = mesh.from_cells(cons_cells)
+        self.hh.set_grid_device(u, u, mesh.grid_records())
+        self.head, self.num, self.ilevel = 1, mesh.noct, mesh.level
+        self.hh.insert_hash_all(1, mesh.noct)
+        self.hh.build_nbor(1, mesh.noct, mesh.level)
+        self.t = 0.0
+        self.dtold = 0.0
+        self.nstep = 0
+
+    def courant(self):
+        return self.hh.cmpdt(self.head, self.num, self.mesh.dx, self.p)
+
+    def step(self, dt=None, tout=None, nlanding=0, precision=0):
+        """One time step in the reference's order; returns dt used."""
+        c = self.courant()
+        if dt is None:
+            dt = newdt(self.mesh.dx, self.p, c["dt"], self.t, tout, nlanding, self.dtold)
+        self.hh.set_unew(self.head, self.num)
+        self.hh.godunov(self.head, self.num, self.ilevel, self.ilevel, self.ilevel, dt, self.mesh.dx, self.p,
+                        precision=precision)
+        self.hh.set_uold(self.head, self.num)
+        self.t += dt
+        self.dtold = dt
+        self.nstep += 1
+        return dt
+
+    def run_device(self, nsteps, dt_fixed=0.0, precision=0, use_graph=False, gravity=(0.0, 0.0, 0.0)):
+        """nsteps device-resident steps (dt reduced on the device)."""
+        g = np.ascontiguousarray(gravity, dtype=np.float64)
+        call("rh_set_precision", int(precision))
+        dt_last = ctypes.c_double()
+        t_tot = ctypes.c_double()
+        sums = np.zeros(4)
+        call("rh_level_run", self.head, self.num, self.ilevel, self.mesh.dx, self.p.gamma, self.p.smallr,
+             self.p.smallc ** 2, self.p.courant_factor, ptr(g), self.p.slope_type, self.p.riemann_id,
+             float(dt_fixed), int(nsteps), int(use_graph), ctypes.byref(dt_last), ctypes.byref(t_tot), ptr(sums))
+        self.t += t_tot.value
+        self.nstep += nsteps
+        return dt_last.value, t_tot.value, sums
+
+    def time_godunov(self, dt, nlaunch, precision=0):
+        g = np.zeros(3)
+        call("rh_set_precision", int(precision))
+        ms = ctypes.c_float()
+        call("rh_time_godunov", self.head, self.num, self.ilevel, self.mesh.dx, self.p.gamma, self.p.smallr,
+             self.p.smallc ** 2, self.p.courant_factor, ptr(g), self.p.slope_type, self.p.riemann_id, float(dt),
+             int(nlaunch), ctypes.byref(ms))
+        return ms.value
+
+    def state(self):
+        return self.hh.transfer_grid_host(self.mesh.noct)
+
+    def cells(self):
+        return self.mesh.to_cells(self.state())

@@ -1,0 +1,132 @@
+"""Oct meshes in the reference layout.
+
+An oct holds 2^ndim cells; a level is a contiguous run of octs; hydro arrays
+are u(1:twotondim, 1:nvar, 1:ngrid) in Fortran order, i.e. numpy arrays of
+shape (ngrid, nvar, twotondim) in C order (amr/amr_commons.f90:816-819,
+amr/init_amr.f90:143).  Oct keys (ckey) are integer oct coordinates at the
+oct's level, 0 <= ckey < 2^(lev-1) in the full box.
+"""
+import numpy as np
+
+NVAR = 5
+
+# type(oct) of amr/oct_commons.f90:5-10 (nhilbert=1)
+OCT3 = np.dtype([("hkey", "<i8"), ("ckey", "<i4", (3,)), ("refined", "u1", (8,)), ("lev", "<i4")])
+OCT1 = np.dtype([("hkey", "<i8"), ("ckey", "<i4", (1,)), ("refined", "u1", (2,)), ("pad", "u1", (2,)),
+                 ("lev", "<i4"), ("pad2", "<i4")])
+assert OCT3.itemsize == 32 and OCT1.itemsize == 24
+
+
+def _part1by2(x):
+    x = x.astype(np.uint64) & np.uint64(0x1FFFFF)
+    x = (x | (x << np.uint64(32))) & np.uint64(0x1F00000000FFFF)
+    x = (x | (x << np.uint64(16))) & np.uint64(0x1F0000FF0000FF)
+    x = (x | (x << np.uint64(8))) & np.uint64(0x100F00F00F00F00F)
+    x = (x | (x << np.uint64(4))) & np.uint64(0x10C30C30C30C30C3)
+    x = (x | (x << np.uint64(2))) & np.uint64(0x1249249249249249)
+    return x
+
+
+def morton3(ix, iy, iz):
+    return _part1by2(ix) | (_part1by2(iy) << np.uint64(1)) | (_part1by2(iz) << np.uint64(2))
+
+
+def level_params(nlevelmax, ndim, box_octs_at=None):
+    """ckey_max(l), key_off(l), box_ckey_min/max(1:3,l) for l = 1..nlevelmax+1.
+
+    key_off(1)=1, key_off(l)=key_off(l-1)+hkey_max(l-1) as metal_runner.f90:26-30
+    with hkey_max(l) = (2^(l-1))^ndim keys at level l.
+    """
+    L = nlevelmax + 1
+    ckey_max = np.array([2 ** (l - 1) for l in range(1, L + 1)], dtype=np.int32)
+    key_off = np.zeros(L, dtype=np.int64)
+    key_off[0] = 1
+    for l in range(1, L):
+        key_off[l] = key_off[l - 1] + int(ckey_max[l - 1]) ** ndim
+    box_min = np.zeros((L, 3), dtype=np.int32)
+    box_max = np.ones((L, 3), dtype=np.int32)
+    for l in range(L):
+        box_max[l, :ndim] = ckey_max[l]
+    return ckey_max, key_off, box_min, box_max
+
+
+class UniformMesh:
+    """One uniform level of 2^(level-1) octs per dimension, octs in Morton order."""
+
+    def __init__(self, ndim, level, boxlen=1.0):
+        assert ndim in (1, 3)
+        self.ndim = ndim
+        self.level = level
+        self.boxlen = boxlen
+        self.n = 2 ** (level - 1)             # octs per dimension
+        self.ncell = 2 * self.n               # cells per dimension
+        self.ttd = 2 ** ndim
+        self.dx = boxlen / 2 ** level
+        n = self.n
+        if ndim == 1:
+            self.ckey = np.arange(n, dtype=np.int32)[:, None]
+        else:
+            ii = np.arange(n, dtype=np.int64)
+            iz, iy, ix = np.meshgrid(ii, ii, ii, indexing="ij")
+            ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
+            order = np.argsort(morton3(ix, iy, iz), kind="stable")
+            self.ckey = np.stack([ix[order], iy[order], iz[order]], axis=1).astype(np.int32)
+        self.noct = self.ckey.shape[0]
+        # cell (i[,j,k]) of the level -> (oct index 0-based, cell index in oct)
+        self._build_maps()
+
+    def _build_maps(self):
+        n, nd = self.n, self.ndim
+        if nd == 1:
+            octmap = np.zeros(n, dtype=np.int64)
+            octmap[self.ckey[:, 0]] = np.arange(self.noct)
+            self.octmap = octmap
+        else:
+            octmap = np.zeros((n, n, n), dtype=np.int64)  # [iz,iy,ix]
+            octmap[self.ckey[:, 2], self.ckey[:, 1], self.ckey[:, 0]] = np.arange(self.noct)
+            self.octmap = octmap
+
+    def grid_records(self):
+        if self.ndim == 3:
+            g = np.zeros(self.noct, dtype=OCT3)
+            g["ckey"] = self.ckey
+        else:
+            g = np.zeros(self.noct, dtype=OCT1)
+            g["ckey"][:, 0] = self.ckey[:, 0]
+        g["lev"] = self.level
+        return g
+
+    def from_cells(self, cells):
+        """cells: (nvar, N[,N,N]) array indexed [var, (k, j,) i] -> uold (noct, nvar, ttd)."""
+        nv = cells.shape[0]
+        u = np.zeros((self.noct, nv, self.ttd), dtype=np.float64)
+        if self.ndim == 1:
+            c = cells.reshape(nv, self.n, 2)                      # [v, ox, i2]
+            u[self.octmap] = np.transpose(c, (1, 0, 2))
+        else:
+            n = self.n
+            c = cells.reshape(nv, n, 2, n, 2, n, 2)              # [v, oz, k2, oy, j2, ox, i2]
+            c = np.transpose(c, (1, 3, 5, 0, 2, 4, 6))             # [oz, oy, ox, v, k2, j2, i2]
+            c = c.reshape(n, n, n, nv, 8)                          # ind = i2 + 2 j2 + 4 k2
+            u[self.octmap.ravel()] = c.reshape(n ** 3, nv, 8)
+        return u
+
+    def to_cells(self, u):
+        nv = u.shape[1]
+        if self.ndim == 1:
+            c = u[self.octmap]                                     # [ox, v, i2]
+            return np.transpose(c, (1, 0, 2)).reshape(nv, self.ncell)
+        n = self.n
+        c = u[self.octmap.ravel()].reshape(n, n, n, nv, 2, 2, 2)   # [oz,oy,ox,v,k2,j2,i2]
+        c = np.transpose(c, (3, 0, 4, 1, 5, 2, 6))                 # [v, oz, k2, oy, j2, ox, i2]
+        return c.reshape(nv, self.ncell, self.ncell, self.ncell)
+
+    def level_params(self, nlevelmax=None):
+        return level_params(nlevelmax or self.level, self.ndim)
+
+
+def prim_to_cons(rho, vel, p, gamma):
+    """(rho, (vx,vy,vz), p) cell arrays -> conservative (5, ...) array."""
+    vx, vy, vz = vel
+    ekin = 0.5 * rho * (vx ** 2 + vy ** 2 + vz ** 2)
+    return np.stack([rho, rho * vx, rho * vy, rho * vz, p / (gamma - 1.0) + ekin])
