@@ -33,11 +33,17 @@ int rh_timer_stop(float *ms);
  * set_uold folded in (unew = uold + dF, then the two arrays swap roles).
  * dt_fixed > 0 imposes dt instead of the Courant value.  dt_last, t_total and
  * the conservation sums of the last Courant pass are returned when non-NULL.
- * use_graph != 0 replays the step pair as a captured hipGraph. */
+ * use_graph != 0 replays the step pair as a captured hipGraph.  kernel_ms, when
+ * non-NULL (and use_graph == 0), receives the mean duration of the Godunov
+ * launches, timed with HIP events on the library stream inside the steps. */
 int rh_level_run(int head_idx, int num_octs, int ilevel, double dx, double gamma, double smallr,
                  double smallc2, double courant_factor, const double g[3], int slope, int riemann,
                  double dt_fixed, int nsteps, int use_graph, double *dt_last, double *t_total,
-                 double sums[4]);
+                 double sums[4], float *kernel_ms);
+
+/* A plain device copy of nbytes (multiple of 32) with a vector-load kernel,
+ * timed with HIP events: the calibration kernel for HBM byte counters. */
+int rh_copy_probe(int64_t nbytes, float *ms);
 
 /* Kernel-only timing helper: nlaunch Godunov launches of the level (fused mode,
  * dt fixed), timed with HIP events on the library stream; ms_per_launch is the

@@ -13,7 +13,8 @@ SIGNATURES = {
     "rh_get_brick_shape": [_ip, _ip, _ip],
     "rh_timer_start": [],
     "rh_timer_stop": [_fp],
-    "rh_level_run": [_c, _c, _c, _d, _d, _d, _d, _d, _dp, _c, _c, _d, _c, _c, _dp, _dp, _dp],
+    "rh_level_run": [_c, _c, _c, _d, _d, _d, _d, _d, _dp, _c, _c, _d, _c, _c, _dp, _dp, _dp, _fp],
+    "rh_copy_probe": [ctypes.c_int64, _fp],
     "rh_time_godunov": [_c, _c, _c, _d, _d, _d, _d, _d, _dp, _c, _c, _d, _c, _fp],
 }
 

@@ -20,8 +20,33 @@ FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Wall", "-Wno-unused-function"]
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FSRC = os.path.join(HERE, "fortran")
+
+
 def lib_path():
     return os.path.join(LIBDIR, LIBNAME)
+
+
+def fortran_driver_path():
+    return os.path.join(LIBDIR, "sod_driver")
+
+
+def build_fortran():
+    """Compile the Fortran binding and the Sod driver against the library
+    (flang from ROCm); skipped when flang is not installed."""
+    if not os.path.exists(FLANG):
+        return None
+    mod = os.path.join(LIBDIR, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    out = fortran_driver_path()
+    cmd = [FLANG, "-O2", "-J", mod, "-o", out, os.path.join(FSRC, "hip_interface.f90"),
+           os.path.join(FSRC, "sod_driver.f90"), "-L" + LIBDIR, "-lramses2_hip", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("flang failed building the Fortran driver")
+    return out
 
 
 def build(verbose=False):
@@ -38,6 +63,7 @@ def build(verbose=False):
     if verbose:
         sys.stderr.write(res.stderr)
     os.replace(tmp, out)
+    build_fortran()
     return out
 
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/ramses2_hip.h"
 #include "../../include/ramses2_hip_ext.h"
@@ -114,7 +115,7 @@ int fill_args1(int head_idx, int num_octs, int ilevel, double dx, double gamma, 
 // One fused step: courant (dt to the device) then godunov uold -> unew.
 hipError_t enqueue_step(int head_idx, int num_octs, double dx, double gamma, double smallr, double smallc2,
                         double courant_factor, const double g[3], double dt_fixed, GodArgs *a3, God1Args *a1,
-                        const LevelBricks *lb)
+                        const LevelBricks *lb, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     State &s = st();
     const double gz[3] = {0, 0, 0};
@@ -124,16 +125,26 @@ hipError_t enqueue_step(int head_idx, int num_octs, double dx, double gamma, dou
                                       s.red_out, s.dt_dev, s.stream, s.t_dev);
         if (e != hipSuccess) return e;
     }
+    hipError_t e;
+    if (e0) (void)hipEventRecord(e0, s.stream);
     if (a3) {
         a3->uold = s.uold;
         a3->unew = s.unew;
         a3->ubase = s.uold;
-        return launch_godunov3d(*a3, lb->bx, lb->by, lb->bz, s.precision, s.stream);
+        e = launch_godunov3d(*a3, lb->bx, lb->by, lb->bz, s.precision, s.stream);
+    } else {
+        a1->uold = s.uold;
+        a1->unew = s.unew;
+        a1->ubase = s.uold;
+        e = launch_godunov1d(*a1, s.precision, s.stream);
     }
-    a1->uold = s.uold;
-    a1->unew = s.unew;
-    a1->ubase = s.uold;
-    return launch_godunov1d(*a1, s.precision, s.stream);
+    if (e1) (void)hipEventRecord(e1, s.stream);
+    return e;
+}
+
+__global__ void k_copy_probe(const double4 *__restrict__ a, double4 *__restrict__ b, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
 }
 
 }  // namespace
@@ -186,7 +197,7 @@ int rh_timer_stop(float *ms)
 
 int rh_level_run(int head_idx, int num_octs, int ilevel, double dx, double gamma, double smallr, double smallc2,
                  double courant_factor, const double g[3], int slope, int riemann, double dt_fixed, int nsteps,
-                 int use_graph, double *dt_last, double *t_total, double sums[4])
+                 int use_graph, double *dt_last, double *t_total, double sums[4], float *kernel_ms)
 {
     State &s = st();
     if (!s.inited || !s.uold) return efail(RH_ERR_STATE, "library not initialised / mesh not allocated");
@@ -235,10 +246,28 @@ int rh_level_run(int head_idx, int num_octs, int ilevel, double dx, double gamma
             std::swap(s.uold, s.unew);
         }
     } else {
+        // per-step events around the Godunov launch (kernel time inside the timed steps)
+        std::vector<hipEvent_t> ev;
+        if (kernel_ms) {
+            ev.resize(2 * (size_t)nsteps);
+            for (auto &e : ev) RH_HIP_CHECK(hipEventCreate(&e));
+        }
         for (int k = 0; k < nsteps; k++) {
             RH_HIP_CHECK(enqueue_step(head_idx, num_octs, dx, gamma, smallr, smallc2, courant_factor, g, dt_fixed,
-                                      s.ndim == 3 ? &a3 : nullptr, s.ndim == 3 ? nullptr : &a1, lb));
+                                      s.ndim == 3 ? &a3 : nullptr, s.ndim == 3 ? nullptr : &a1, lb,
+                                      kernel_ms ? ev[2 * k] : nullptr, kernel_ms ? ev[2 * k + 1] : nullptr));
             std::swap(s.uold, s.unew);
+        }
+        if (kernel_ms) {
+            RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+            double tot = 0;
+            for (int k = 0; k < nsteps; k++) {
+                float t = 0;
+                RH_HIP_CHECK(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]));
+                tot += t;
+            }
+            kernel_ms[0] = nsteps ? (float)(tot / nsteps) : 0.0f;
+            for (auto &e : ev) (void)hipEventDestroy(e);
         }
     }
     // unew mirrors uold as after set_uold
@@ -253,6 +282,29 @@ int rh_level_run(int head_idx, int num_octs, int ilevel, double dx, double gamma
     if (t_total) *t_total = dt_fixed > 0.0 ? dt_fixed * nsteps : h[0];
     if (sums)
         for (int i = 0; i < 4; i++) sums[i] = s.h_red[i];
+    return 0;
+}
+
+int rh_copy_probe(int64_t nbytes, float *ms)
+{
+    State &s = st();
+    if (!s.inited) return efail(RH_ERR_STATE, "rh_init has not been called");
+    if (nbytes <= 0 || nbytes % 32) return efail(RH_ERR_ARG, "nbytes must be a positive multiple of 32");
+    void *a = nullptr, *b = nullptr;
+    RH_HIP_CHECK(hipMalloc(&a, (size_t)nbytes));
+    RH_HIP_CHECK(hipMalloc(&b, (size_t)nbytes));
+    RH_HIP_CHECK(hipMemsetAsync(a, 0, (size_t)nbytes, s.stream));
+    const long n = (long)(nbytes / 32);
+    RH_HIP_CHECK(hipEventRecord(s.ev0, s.stream));
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s.stream, (const double4 *)a, (double4 *)b, n);
+    RH_HIP_CHECK(hipGetLastError());
+    RH_HIP_CHECK(hipEventRecord(s.ev1, s.stream));
+    RH_HIP_CHECK(hipEventSynchronize(s.ev1));
+    float t = 0;
+    RH_HIP_CHECK(hipEventElapsedTime(&t, s.ev0, s.ev1));
+    if (ms) *ms = t;
+    (void)hipFree(a);
+    (void)hipFree(b);
     return 0;
 }
 

@@ -192,19 +192,24 @@ class UniformLevel:
         self.nstep += 1
         return dt
 
-    def run_device(self, nsteps, dt_fixed=0.0, precision=0, use_graph=False, gravity=(0.0, 0.0, 0.0)):
-        """nsteps device-resident steps (dt reduced on the device)."""
+    def run_device(self, nsteps, dt_fixed=0.0, precision=0, use_graph=False, gravity=(0.0, 0.0, 0.0),
+                   time_kernel=False):
+        """nsteps device-resident steps (dt reduced on the device).  Returns
+        (dt_last, t_total, sums, godunov_ms) where godunov_ms is the mean
+        Godunov launch duration (HIP events) when time_kernel is set."""
         g = np.ascontiguousarray(gravity, dtype=np.float64)
         call("rh_set_precision", int(precision))
         dt_last = ctypes.c_double()
         t_tot = ctypes.c_double()
         sums = np.zeros(4)
+        kms = ctypes.c_float(-1.0)
         call("rh_level_run", self.head, self.num, self.ilevel, self.mesh.dx, self.p.gamma, self.p.smallr,
              self.p.smallc ** 2, self.p.courant_factor, ptr(g), self.p.slope_type, self.p.riemann_id,
-             float(dt_fixed), int(nsteps), int(use_graph), ctypes.byref(dt_last), ctypes.byref(t_tot), ptr(sums))
+             float(dt_fixed), int(nsteps), int(use_graph), ctypes.byref(dt_last), ctypes.byref(t_tot), ptr(sums),
+             ctypes.byref(kms) if time_kernel else None)
         self.t += t_tot.value
         self.nstep += nsteps
-        return dt_last.value, t_tot.value, sums
+        return dt_last.value, t_tot.value, sums, (kms.value if time_kernel else None)
 
     def time_godunov(self, dt, nlaunch, precision=0):
         g = np.zeros(3)

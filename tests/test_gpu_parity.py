@@ -164,7 +164,7 @@ def test_device_run_matches_dropin_calls():
         a.step()
     ua = a.state()
     b = UniformLevel(mesh, cells, p)
-    b.run_device(6, use_graph=False)
+    b.run_device(6, use_graph=False, time_kernel=True)
     assert np.array_equal(b.state(), ua)
     c = UniformLevel(mesh, cells, p)
     c.run_device(6, use_graph=True)
