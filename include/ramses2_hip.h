@@ -127,6 +127,21 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
                double gamma, double smallr, double smallc2, double dt, double dx, double smax_cap,
                int slope, int riemann, const double g[3]);
 
+/* Replaces mtl_upload (metal/metal_interface.f90:520-527), i.e. upload_fine
+ * (hydro/upload.f90:74-283): average each oct of the fine level
+ * head_idx..head_idx+num_octs-1 into its parent cell of uold;
+ * internal_energy != 0 averages the internal rather than the total energy
+ * (interpol_var == 1).  rh_build_nbor of the fine level must have run. */
+int rh_upload(int head_idx, int num_octs, int internal_energy, double gamma, double smallr, double smallc2);
+
+/* &REFINE_PARAMS interpol_var (0: rho, rho u, E; 1: rho, rho u, rho e) and
+ * interpol_type (0 injection, 1 MinMod, 2 MonCen, 3 central) used to fill the
+ * ghost octs of missing coarse-fine neighbours (interpol_hydro,
+ * hydro/interpol_hydro.f90:5-113, as godfine1 calls it at
+ * hydro/godunov_fine.f90:555).  No Metal counterpart: the CUDA backend reads
+ * them from its module variables (gpu/gpu_refine.cuf:2231 make_cache_octs). */
+int rh_set_interpolation(int interpol_var, int interpol_type);
+
 /* Replaces mtl_device_sync (metal/metal_interface.f90:529-531). */
 int rh_device_sync(void);
 

@@ -84,6 +84,8 @@ def lib():
         L.ora_godunov_fine.restype = ctypes.c_int
         L.ora_courant_fine.argtypes = [ctypes.POINTER(Mesh), ctypes.c_int,
                                        ctypes.POINTER(Params), ctypes.c_int, _dp]
+        L.ora_upload_fine.argtypes = [ctypes.POINTER(Mesh), ctypes.c_int, ctypes.c_double]
+        L.ora_upload_fine.restype = ctypes.c_int
         L.ora_set_unew.argtypes = [ctypes.POINTER(Mesh), ctypes.c_int]
         L.ora_set_uold.argtypes = [ctypes.POINTER(Mesh), ctypes.c_int]
         L.ora_godunov_uniform_omp.argtypes = [ctypes.POINTER(Mesh), ctypes.c_int,
@@ -211,6 +213,11 @@ class OracleMesh:
 
     def set_unew(self, ilevel):
         lib().ora_set_unew(ctypes.byref(self.m), ilevel)
+
+    def upload_fine(self, ilevel, smallr=1e-10):
+        rc = lib().ora_upload_fine(ctypes.byref(self.m), ilevel, smallr)
+        if rc != 0:
+            raise RuntimeError("ora_upload_fine: parent cell missing")
 
     def set_uold(self, ilevel):
         lib().ora_set_uold(ctypes.byref(self.m), ilevel)

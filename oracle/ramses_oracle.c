@@ -1082,6 +1082,63 @@ void ora_set_uold(const ora_mesh_t *m, int ilevel)
     memcpy(m->uold + (size_t)(head - 1) * per, m->unew + (size_t)(head - 1) * per, sizeof(double) * per * noct);
 }
 
+/* upload_fine, hydro/upload.f90:74-283 */
+int ora_upload_fine(const ora_mesh_t *m, int ilevel, double smallr)
+{
+    int head, noct, fhead, fnoct;
+    level_range(m, ilevel, &head, &noct);
+    if (ilevel >= m->nlevelmax) return 0;
+    level_range(m, ilevel + 1, &fhead, &fnoct);
+    int ndim = m->ndim, nvar = m->nvar, ttd = 1 << ndim;
+    for (int ioct = head; ioct < head + noct; ioct++)
+        for (int ivar = 1; ivar <= nvar; ivar++)
+            for (int ind = 1; ind <= ttd; ind++)
+                if (m->refined[(size_t)(ioct - 1) * 8 + (ind - 1)]) UOCT(m->uold, nvar, ioct, ivar, ind, ttd) = 0.0;
+    omap_t h;
+    omap_build(&h, m);
+    lookup_t L = {m, &h, ttd};
+    int rc = 0;
+    for (int ioct = fhead; ioct < fhead + fnoct; ioct++) {
+        int64_t kf[3] = {0, 0, 0};
+        int icell = 1;
+        for (int d = 0; d < ndim; d++) {
+            int64_t k = m->ckey[(size_t)(ioct - 1) * 3 + d];
+            kf[d] = fdiv2(k);
+            icell += (1 << d) * (int)(k - 2 * kf[d]);
+        }
+        ghost_t gp;
+        int igrid = get_grid(&L, ilevel, kf, &gp);
+        if (igrid <= 0) {
+            rc = 1;
+            break;
+        }
+        for (int ivar = 1; ivar <= nvar; ivar++) {
+            double average = 0.0;
+            for (int ind = 1; ind <= ttd; ind++) average = average + UOCT(m->uold, nvar, ioct, ivar, ind, ttd);
+            UOCT(m->uold, nvar, igrid, ivar, icell, ttd) = average / (double)ttd;
+        }
+        if (m->interpol_var == 1) {
+            double average = 0.0;
+            for (int ind = 1; ind <= ttd; ind++) {
+                double ekin = 0.0;
+                for (int idim = 1; idim <= 3; idim++) {
+                    double mm = UOCT(m->uold, nvar, ioct, idim + 1, ind, ttd);
+                    ekin = ekin + 0.5 * (mm * mm) / dmax(UOCT(m->uold, nvar, ioct, 1, ind, ttd), smallr);
+                }
+                average = average + UOCT(m->uold, nvar, ioct, 5, ind, ttd) - ekin - 0.0 - 0.0;
+            }
+            double ekin = 0.0;
+            for (int idim = 1; idim <= 3; idim++) {
+                double mm = UOCT(m->uold, nvar, igrid, idim + 1, icell, ttd);
+                ekin = ekin + 0.5 * (mm * mm) / dmax(UOCT(m->uold, nvar, igrid, 1, icell, ttd), smallr);
+            }
+            UOCT(m->uold, nvar, igrid, 5, icell, ttd) = average / (double)ttd + ekin + 0.0 + 0.0;
+        }
+    }
+    omap_free(&h);
+    return rc;
+}
+
 /* courant_fine, hydro/courant_fine.f90:57-168 */
 void ora_courant_fine(const ora_mesh_t *m, int ilevel, const ora_params_t *p, int nthreads, double *out)
 {

@@ -92,6 +92,10 @@ void ora_courant_fine(const ora_mesh_t *m, int ilevel, const ora_params_t *p,
 void ora_set_unew(const ora_mesh_t *m, int ilevel);
 void ora_set_uold(const ora_mesh_t *m, int ilevel);
 
+/* upload_fine, hydro/upload.f90:74-283: zero the refined cells of level
+ * ilevel, then average each oct of level ilevel+1 into its parent cell. */
+int ora_upload_fine(const ora_mesh_t *m, int ilevel, double smallr);
+
 /* Uniform-level fast path for the CPU baseline: same arithmetic as
  * ora_godunov_fine on a periodic uniform level whose octs are addressed
  * through a dense (ix,iy,iz) -> oct map, run with OpenMP over octs.          */

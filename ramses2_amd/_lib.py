@@ -34,6 +34,8 @@ SIGNATURES = {
     "rh_cmpdt": [_c, _c, _d, _d, _d, _d, _d, _dp, _dp, _dp, _dp, _dp, _dp],
     "rh_godunov": [_c, _c, _c, _c, _c, _c, _d, _d, _d, _d, _d, _d, _c, _c, _dp],
     "rh_device_sync": [],
+    "rh_upload": [_c, _c, _c, _d, _d, _d],
+    "rh_set_interpolation": [_c, _c],
     "rh_last_error": None,
 }
 
@@ -49,7 +51,7 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    path = _build.lib_path()
+    path = os.environ.get("RAMSES2_HIP_LIB") or _build.lib_path()
     if not os.path.exists(path):
         raise ImportError("ramses2_amd: %s is not built; run ramses2_amd/build.py "
                           "(or __graft_entry__.build())" % path)

@@ -49,13 +49,13 @@ def build_fortran():
     return out
 
 
-def build(verbose=False):
+def build(verbose=False, out=None, defines=()):
     """Compile every HIP source into the shared library (always recompiles)."""
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    out = lib_path()
+    out = out or lib_path()
     tmp = out + ".tmp%d" % os.getpid()
-    cmd = [HIPCC] + FLAGS + ["-o", tmp] + srcs
+    cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + ["-o", tmp] + srcs
     res = subprocess.run(cmd, cwd=CSRC, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
@@ -63,7 +63,8 @@ def build(verbose=False):
     if verbose:
         sys.stderr.write(res.stderr)
     os.replace(tmp, out)
-    build_fortran()
+    if out == lib_path():
+        build_fortran()
     return out
 
 

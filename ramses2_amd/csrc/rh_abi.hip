@@ -154,6 +154,130 @@ __global__ void k_build_bricks(const int32_t *__restrict__ org, int nbricks, int
     win[i] = resolve_oct(c, org[3 * b] + ox - 1, org[3 * b + 1] + oy - 1, org[3 * b + 2] + oz - 1);
 }
 
+// Count the in-domain neighbour slots of the level that no oct occupies.
+__global__ void k_count_missing(const int32_t *__restrict__ nbor, long n, int *counter)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && nbor[i] == 0) atomicAdd(counter, 1);
+}
+
+// For every missing neighbour slot, insert its (wrapped) key into the device
+// hash once and give it a ghost oct index in the cache region
+// ngridmax+1..ngridmax+ncachemax (the role of make_cache_octs /
+// insert_hash_cache, gpu/gpu_refine.cuf:937,2231).  counter[0] counts ghosts,
+// counter[1] flags overflow.
+__global__ void k_make_ghosts(const int32_t *__restrict__ ckey, const int32_t *__restrict__ nbor, int head, int noct,
+                              ResolveCtx c, int nslot, int ilevel, int ngridmax, int ghost_base, int ghost_cap,
+                              uint64_t *keys, int32_t *vals, int *counter, int32_t *gkey, int32_t *g_ckey,
+                              int32_t *g_lev, uint8_t *g_ref)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)noct * nslot || nbor[i] != 0) return;
+    const int o = head - 1 + (int)(i / nslot);
+    const int s = (int)(i % nslot);
+    int k[3] = {ckey[3 * o] + s % 3 - 1, ckey[3 * o + 1] + (s / 3) % 3 - 1, ckey[3 * o + 2] + s / 9 - 1};
+    for (int d = 0; d < c.ndim; d++) {
+        if (c.periodic[d]) {
+            const int n = c.boxmax[d] - c.boxmin[d];
+            int r = (k[d] - c.boxmin[d]) % n;
+            if (r < 0) r += n;
+            k[d] = c.boxmin[d] + r;
+        }
+        if (k[d] < c.boxmin[d] || k[d] >= c.boxmax[d]) return;  // boundary ghosts are synthesised, not stored
+    }
+    const uint64_t key = oct_key(c.koff, c.nx, c.ndim, k[0], k[1], k[2]);
+    uint64_t h = hmix(key) & (c.cap - 1);
+    for (uint64_t p = 0; p < c.cap; p++) {
+        unsigned long long prev = atomicCAS((unsigned long long *)&keys[h], (unsigned long long)kEmpty,
+                                            (unsigned long long)key);
+        if (prev == key) return;  // another slot created it
+        if (prev == kEmpty) {
+            const int gi = atomicAdd(&counter[0], 1);
+            if (ghost_base + gi >= ghost_cap) {
+                atomicOr(&counter[1], 1);
+                vals[h] = 0;
+                return;
+            }
+            const int idx = ngridmax + 1 + ghost_base + gi;  // 1-based oct index
+            vals[h] = idx;
+            gkey[3 * (ghost_base + gi) + 0] = k[0];
+            gkey[3 * (ghost_base + gi) + 1] = k[1];
+            gkey[3 * (ghost_base + gi) + 2] = k[2];
+            g_ckey[3 * (idx - 1) + 0] = k[0];
+            g_ckey[3 * (idx - 1) + 1] = k[1];
+            g_ckey[3 * (idx - 1) + 2] = k[2];
+            g_lev[idx - 1] = ilevel;
+            for (int q = 0; q < 8; q++) g_ref[8 * (size_t)(idx - 1) + q] = 0;
+            return;
+        }
+        h = (h + 1) & (c.cap - 1);
+    }
+    atomicOr(&counter[1], 2);
+}
+
+// Parent cell (get_parent_cell, amr/nbors_utils.f90:222-249) and the 2*ndim
+// neighbouring parent cells (get_twondim_nbor_parent_cell, :149-217) of each
+// new ghost: the stencil interpol_hydro reads.  counter[2] flags a missing
+// parent ("GODUNOV: parent_cell should exist", hydro/godunov_fine.f90:491).
+__global__ void k_ghost_sources(const int32_t *__restrict__ gkey, int first_gi, int n, ResolveCtx fine,
+                                ResolveCtx coarse, int32_t *gparent, int32_t *gsrc, int32_t *gsrckey, int *counter)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int gi = first_gi + j;
+    const int nd = fine.ndim;
+    int k[3] = {gkey[3 * gi], gkey[3 * gi + 1], gkey[3 * gi + 2]};
+    for (int slot = 0; slot <= 2 * nd; slot++) {
+        int kn[3] = {k[0], k[1], k[2]};
+        if (slot > 0) {
+            const int d = (slot - 1) / 2;
+            kn[d] += (slot % 2 == 1) ? -1 : 1;
+            if (fine.periodic[d]) {
+                const int nn = fine.boxmax[d] - fine.boxmin[d];
+                int r = (kn[d] - fine.boxmin[d]) % nn;
+                if (r < 0) r += nn;
+                kn[d] = fine.boxmin[d] + r;
+            }
+        }
+        int kf[3] = {0, 0, 0}, ind = 1;
+        for (int d = 0; d < nd; d++) {
+            kf[d] = kn[d] / 2;  // Fortran integer division
+            ind += (1 << d) * (kn[d] - 2 * kf[d]);
+        }
+        const int code = resolve_oct(coarse, kf[0], kf[1], kf[2]);
+        gsrc[(gi * 7 + slot) * 2] = code;
+        gsrc[(gi * 7 + slot) * 2 + 1] = ind;
+        gsrckey[(gi * 7 + slot) * 3] = kf[0];
+        gsrckey[(gi * 7 + slot) * 3 + 1] = kf[1];
+        gsrckey[(gi * 7 + slot) * 3 + 2] = kf[2];
+        if (slot == 0) {
+            gparent[2 * gi] = code;
+            gparent[2 * gi + 1] = ind;
+            if (code <= 0) atomicOr(&counter[2], 1);
+        } else if (code == 0) {
+            atomicOr(&counter[2], 2);
+        }
+    }
+}
+
+// father(oct) = parent cell of each oct of the level (update_father_array,
+// gpu/gpu_refine.cuf:1964).
+__global__ void k_build_father(const int32_t *__restrict__ ckey, int head, int noct, ResolveCtx coarse, int ndim,
+                               int32_t *father)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= noct) return;
+    const int o = head - 1 + i;
+    int kf[3] = {0, 0, 0}, ind = 1;
+    for (int d = 0; d < ndim; d++) {
+        kf[d] = ckey[3 * o + d] / 2;
+        ind += (1 << d) * (ckey[3 * o + d] - 2 * kf[d]);
+    }
+    const int code = resolve_oct(coarse, kf[0], kf[1], kf[2]);
+    father[2 * i] = code > 0 ? code : 0;
+    father[2 * i + 1] = ind;
+}
+
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
@@ -177,7 +301,10 @@ static void free_level_tables()
         (void)hipFree(kv.second.org);
     }
     g_state.bricks.clear();
-    for (auto &kv : g_state.nbors) (void)hipFree(kv.second.nbor);
+    for (auto &kv : g_state.nbors) {
+        (void)hipFree(kv.second.nbor);
+        (void)hipFree(kv.second.father);
+    }
     g_state.nbors.clear();
 }
 
@@ -239,6 +366,7 @@ int rh_init(void)
     s.red_cap = 4096;
     RH_HIP_CHECK(hipMalloc(&s.red_part, (size_t)s.red_cap * 4 * sizeof(double)));
     RH_HIP_CHECK(hipMalloc(&s.dt_dev, 2 * sizeof(double)));
+    RH_HIP_CHECK(hipMalloc(&s.d_counter, 4 * sizeof(int)));
     RH_HIP_CHECK(hipMalloc(&s.t_dev, 2 * sizeof(double)));
     RH_HIP_CHECK(hipEventCreate(&s.ev0));
     RH_HIP_CHECK(hipEventCreate(&s.ev1));
@@ -260,6 +388,11 @@ int rh_finalize(void)
     (void)hipFree(s.refined);
     (void)hipFree(s.hkeys);
     (void)hipFree(s.hvals);
+    (void)hipFree(s.ghost_parent);
+    (void)hipFree(s.ghost_src);
+    (void)hipFree(s.ghost_srckey);
+    (void)hipFree(s.ghost_key);
+    (void)hipFree(s.d_counter);
     (void)hipFree(s.red_part);
     (void)hipFree(s.red_out);
     (void)hipHostFree(s.h_red);
@@ -314,6 +447,19 @@ int rh_alloc_amr(int ngridmax, int ncachemax, int nvar, int twotondim, int hash_
     RH_HIP_CHECK(hipMemsetAsync(s.refined, 0, (size_t)s.ngridtot * 8, s.stream));
     RH_HIP_CHECK(hipMemsetAsync(s.hkeys, 0xFF, (size_t)cap * sizeof(uint64_t), s.stream));
     RH_HIP_CHECK(hipMemsetAsync(s.hvals, 0, (size_t)cap * sizeof(int32_t), s.stream));
+    {
+        const size_t ng = (size_t)(ncachemax > 0 ? ncachemax : 1);
+        (void)hipFree(s.ghost_parent);
+        (void)hipFree(s.ghost_src);
+        (void)hipFree(s.ghost_srckey);
+        (void)hipFree(s.ghost_key);
+        if (hipMalloc(&s.ghost_parent, ng * 2 * sizeof(int32_t)) != hipSuccess ||
+            hipMalloc(&s.ghost_src, ng * 14 * sizeof(int32_t)) != hipSuccess ||
+            hipMalloc(&s.ghost_srckey, ng * 21 * sizeof(int32_t)) != hipSuccess ||
+            hipMalloc(&s.ghost_key, ng * 3 * sizeof(int32_t)) != hipSuccess)
+            return fail(RH_ERR_NOMEM, "device allocation failed for the ghost-oct tables");
+        s.nghost_total = 0;
+    }
     s.h_ckey.assign((size_t)s.ngridtot * 3, 0);
     s.h_lev.assign((size_t)s.ngridtot, 0);
     RH_HIP_CHECK(hipStreamSynchronize(s.stream));
@@ -378,8 +524,9 @@ int rh_set_grid_device(const double *uold, const double *unew, const void *bold,
         RH_HIP_CHECK(hipMemcpyAsync(s.lev, s.h_lev.data(), (size_t)ngridmax * sizeof(int32_t), hipMemcpyHostToDevice, s.stream));
         RH_HIP_CHECK(hipMemcpyAsync(s.refined, refb.data(), (size_t)ngridmax * 8, hipMemcpyHostToDevice, s.stream));
         RH_HIP_CHECK(hipStreamSynchronize(s.stream));
-        // mesh changed: tables must be rebuilt
+        // mesh changed: tables and ghost octs must be rebuilt
         free_level_tables();
+        s.nghost_total = 0;
         RH_HIP_CHECK(hipMemsetAsync(s.hkeys, 0xFF, (size_t)s.hash_cap * sizeof(uint64_t), s.stream));
     }
     RH_HIP_CHECK(hipStreamSynchronize(s.stream));
@@ -454,26 +601,84 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
         if (s.h_lev[i] != ilevel) return fail(RH_ERR_MESH, "octs of the range are not all on one level");
     ResolveCtx c = make_resolve(ilevel, box_min, box_max, periodic);
 
-    // per-oct neighbour table (3^ndim slots)
+    // per-oct neighbour table (3^ndim slots), father table and ghost octs
     {
+        int prev_first = 0, prev_n = 0;
         auto it = s.nbors.find(ilevel);
         if (it != s.nbors.end()) {
+            prev_first = it->second.ghost_first;
+            prev_n = it->second.nghost;
             (void)hipFree(it->second.nbor);
+            (void)hipFree(it->second.father);
             s.nbors.erase(it);
         }
         LevelNbor ln;
         ln.ilevel = ilevel;
         ln.head = head_idx;
         ln.noct = num_octs;
+        ln.ghost_first = prev_first;
+        ln.nghost = prev_n;
         const int nslot = s.ndim == 3 ? 27 : 3;
         RH_HIP_CHECK(hipMalloc(&ln.nbor, (size_t)num_octs * nslot * sizeof(int32_t)));
         const long n = (long)num_octs * nslot;
-        hipLaunchKernelGGL(k_build_nbor, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s.stream, s.ckey, head_idx,
-                           num_octs, c, nslot, ln.nbor);
+        const unsigned nblk = (unsigned)((n + 255) / 256);
+        hipLaunchKernelGGL(k_build_nbor, dim3(nblk), dim3(256), 0, s.stream, s.ckey, head_idx, num_octs, c, nslot,
+                           ln.nbor);
         RH_HIP_CHECK(hipGetLastError());
         for (int d = 0; d < 3; d++) {
             ln.boxmin[d] = c.boxmin[d];
             ln.boxmax[d] = c.boxmax[d];
+        }
+        if (ilevel > 1) {
+            ResolveCtx cc = make_resolve(ilevel - 1, &s.box_min[3 * (ilevel - 2)], &s.box_max[3 * (ilevel - 2)],
+                                         periodic);
+            RH_HIP_CHECK(hipMalloc(&ln.father, (size_t)num_octs * 2 * sizeof(int32_t)));
+            hipLaunchKernelGGL(k_build_father, dim3((num_octs + 255) / 256), dim3(256), 0, s.stream, s.ckey,
+                               head_idx, num_octs, cc, s.ndim, ln.father);
+            RH_HIP_CHECK(hipGetLastError());
+            int hc[4] = {0, 0, 0, 0};
+            RH_HIP_CHECK(hipMemsetAsync(s.d_counter, 0, 4 * sizeof(int), s.stream));
+            hipLaunchKernelGGL(k_count_missing, dim3(nblk), dim3(256), 0, s.stream, ln.nbor, n, s.d_counter);
+            RH_HIP_CHECK(hipGetLastError());
+            RH_HIP_CHECK(hipMemcpyAsync(hc, s.d_counter, 4 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
+            RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+            if (hc[0] > 0) {
+                const int base = s.nghost_total;
+                RH_HIP_CHECK(hipMemsetAsync(s.d_counter, 0, 4 * sizeof(int), s.stream));
+                hipLaunchKernelGGL(k_make_ghosts, dim3(nblk), dim3(256), 0, s.stream, s.ckey, ln.nbor, head_idx,
+                                   num_octs, c, nslot, ilevel, s.ngridmax, base, s.ncachemax, s.hkeys, s.hvals,
+                                   s.d_counter, s.ghost_key, s.ckey, s.lev, s.refined);
+                RH_HIP_CHECK(hipGetLastError());
+                RH_HIP_CHECK(hipMemcpyAsync(hc, s.d_counter, 4 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
+                RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+                if (hc[1]) {
+                    s.nbors[ilevel] = ln;
+                    return fail(RH_ERR_MESH, "ncachemax too small for the ghost octs of the level (or hash full)");
+                }
+                const int nnew = hc[0];
+                RH_HIP_CHECK(hipMemsetAsync(s.d_counter, 0, 4 * sizeof(int), s.stream));
+                hipLaunchKernelGGL(k_ghost_sources, dim3((nnew + 127) / 128), dim3(128), 0, s.stream, s.ghost_key,
+                                   base, nnew, c, cc, s.ghost_parent, s.ghost_src, s.ghost_srckey, s.d_counter);
+                RH_HIP_CHECK(hipGetLastError());
+                RH_HIP_CHECK(hipMemcpyAsync(hc, s.d_counter, 4 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
+                RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+                ln.ghost_first = s.ngridmax + 1 + base;
+                ln.nghost = nnew;
+                s.nghost_total += nnew;
+                for (int g = 0; g < nnew; g++) s.h_lev[s.ngridmax + base + g] = ilevel;
+                if (hc[2] & 1) {
+                    s.nbors[ilevel] = ln;
+                    return fail(RH_ERR_MESH, "GODUNOV: parent_cell should exist");
+                }
+                if (hc[2] & 2) {
+                    s.nbors[ilevel] = ln;
+                    return fail(RH_ERR_MESH, "GODUNOV: parent neighbors should exist");
+                }
+                // the ghosts now resolve through the hash
+                hipLaunchKernelGGL(k_build_nbor, dim3(nblk), dim3(256), 0, s.stream, s.ckey, head_idx, num_octs, c,
+                                   nslot, ln.nbor);
+                RH_HIP_CHECK(hipGetLastError());
+            }
         }
         s.nbors[ilevel] = ln;
     }
@@ -605,13 +810,35 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
     if (riemann < 1 || riemann > 3) return fail(RH_ERR_ARG, "riemann must be 1 (llf), 2 (hll) or 3 (hllc)");
     if (slope < 0 || slope > 3) return fail(RH_ERR_ARG, "slope_type must be 0..3");
     if (!(dt > 0) || !(dx > 0)) return fail(RH_ERR_ARG, "dt and dx must be positive");
-    if (ilevel > levelmin) {
-        // coarse-fine refluxing is served by the AMR level path
-        // (rh_godunov rejects it until the ghost-oct pipeline is built)
-        return fail(RH_ERR_STATE, "rh_godunov: AMR levels (ilevel > levelmin) are not supported yet");
-    }
     const double gz[3] = {0, 0, 0};
     const double *gg = g ? g : gz;
+    auto itn = s.nbors.find(ilevel);
+    if (itn == s.nbors.end() || itn->second.head != head_idx || itn->second.noct != num_subgrids)
+        return fail(RH_ERR_STATE, "rh_godunov: rh_build_nbor was not called for this level");
+    const LevelNbor &lnb = itn->second;
+    // ghost octs of the level take their values from the parent level now
+    if (lnb.nghost > 0) {
+        GhostArgs ga;
+        ga.uold = s.uold;
+        ga.refined = s.refined;
+        const int gi0 = lnb.ghost_first - s.ngridmax - 1;
+        ga.src = s.ghost_src + (size_t)gi0 * 14;
+        ga.srckey = s.ghost_srckey + (size_t)gi0 * 21;
+        ga.first = lnb.ghost_first;
+        ga.nghost = lnb.nghost;
+        ga.ndim = s.ndim;
+        ga.interpol_var = s.interpol_var;
+        ga.interpol_type = s.interpol_type;
+        ga.smallr = smallr;
+        for (int d = 0; d < 3; d++) {
+            ga.boxmin[d] = d < s.ndim ? s.box_min[3 * (ilevel - 2) + d] : 0;
+            ga.boxmax[d] = d < s.ndim ? s.box_max[3 * (ilevel - 2) + d] : 1;
+            ga.btype[d][0] = s.btype[d][0];
+            ga.btype[d][1] = s.btype[d][1];
+        }
+        RH_HIP_CHECK(launch_refresh_ghosts(ga, s.stream));
+    }
+    const int reflux = ilevel > levelmin ? 1 : 0;
     if (s.ndim == 3) {
         auto it = s.bricks.find(ilevel);
         if (it == s.bricks.end() || it->second.head != head_idx || it->second.noct != num_subgrids)
@@ -647,12 +874,14 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         a.slope = slope;
         a.riemann = riemann;
         a.xcd_remap = 1;
+        a.reflux = reflux;
+        a.ngridmax = s.ngridmax;
+        a.ghost_parent = s.ghost_parent;
+        a.courant = 0;
+        a.cpart = nullptr;
         RH_HIP_CHECK(launch_godunov3d(a, lb.bx, lb.by, lb.bz, s.precision, s.stream));
     } else {
-        auto it = s.nbors.find(ilevel);
-        if (it == s.nbors.end() || it->second.head != head_idx || it->second.noct != num_subgrids)
-            return fail(RH_ERR_STATE, "rh_godunov: rh_build_nbor was not called for this level");
-        const LevelNbor &ln = it->second;
+        const LevelNbor &ln = lnb;
         God1Args a;
         a.uold = s.uold;
         a.unew = s.unew;
@@ -678,8 +907,37 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         a.g = gg[0];
         a.slope = slope;
         a.riemann = riemann;
+        a.reflux = reflux;
+        a.ngridmax = s.ngridmax;
+        a.ghost_parent = s.ghost_parent;
         RH_HIP_CHECK(launch_godunov1d(a, s.precision, s.stream));
     }
+    return 0;
+}
+
+int rh_set_interpolation(int interpol_var, int interpol_type)
+{
+    if (interpol_var < 0 || interpol_var > 1 || interpol_type < 0 || interpol_type > 3)
+        return fail(RH_ERR_ARG, "interpol_var must be 0/1 and interpol_type 0..3");
+    g_state.interpol_var = interpol_var;
+    g_state.interpol_type = interpol_type;
+    return 0;
+}
+
+int rh_upload(int head_idx, int num_octs, int internal_energy, double gamma, double smallr, double smallc2)
+{
+    if (int r = ensure_mesh()) return r;
+    State &s = g_state;
+    (void)gamma;
+    (void)smallc2;
+    if (num_octs == 0) return 0;
+    int ilevel = 0;
+    if (int r = level_of_range(head_idx, num_octs, ilevel)) return r;
+    auto it = s.nbors.find(ilevel);
+    if (it == s.nbors.end() || it->second.head != head_idx || it->second.noct != num_octs || !it->second.father)
+        return fail(RH_ERR_STATE, "rh_upload: rh_build_nbor was not called for the fine level");
+    RH_HIP_CHECK(launch_upload(s.uold, it->second.father, head_idx, num_octs, s.ndim, internal_energy, smallr,
+                               s.stream));
     return 0;
 }
 

@@ -73,6 +73,11 @@ int fill_args3(int head_idx, int num_octs, int ilevel, double dx, double gamma, 
     a.slope = slope;
     a.riemann = riemann;
     a.xcd_remap = 1;
+    a.reflux = 0;
+    a.ngridmax = s.ngridmax;
+    a.ghost_parent = s.ghost_parent;
+    a.courant = 0;
+    a.cpart = nullptr;
     return 0;
 }
 
@@ -109,16 +114,46 @@ int fill_args1(int head_idx, int num_octs, int ilevel, double dx, double gamma, 
     a.g = g ? g[0] : 0.0;
     a.slope = slope;
     a.riemann = riemann;
+    a.reflux = 0;
+    a.ngridmax = s.ngridmax;
+    a.ghost_parent = s.ghost_parent;
     return 0;
 }
 
-// One fused step: courant (dt to the device) then godunov uold -> unew.
+// Grow the partial-sum buffer to n entries of 4 doubles.
+int ensure_partials(int n)
+{
+    State &s = st();
+    if (n <= s.red_cap) return 0;
+    (void)hipFree(s.red_part);
+    s.red_part = nullptr;
+    RH_HIP_CHECK(hipMalloc(&s.red_part, (size_t)n * 4 * sizeof(double)));
+    s.red_cap = n;
+    return 0;
+}
+
+// One step.  3D with a Courant dt: the Godunov kernel also reduces the
+// Courant condition of the state it writes (dt of the next step), followed by
+// a one-block final reduction; 1D or fixed dt: courant then godunov.
 hipError_t enqueue_step(int head_idx, int num_octs, double dx, double gamma, double smallr, double smallc2,
                         double courant_factor, const double g[3], double dt_fixed, GodArgs *a3, God1Args *a1,
                         const LevelBricks *lb, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     State &s = st();
     const double gz[3] = {0, 0, 0};
+    if (a3 && dt_fixed <= 0.0) {
+        a3->uold = s.uold;
+        a3->unew = s.unew;
+        a3->ubase = s.uold;
+        a3->courant = 1;
+        a3->cpart = s.red_part;
+        if (e0) (void)hipEventRecord(e0, s.stream);
+        hipError_t e = launch_godunov3d(*a3, lb->bx, lb->by, lb->bz, s.precision, s.stream);
+        if (e1) (void)hipEventRecord(e1, s.stream);
+        if (e != hipSuccess) return e;
+        return launch_courant_reduce(s.red_part, lb->nbricks, dx / __builtin_sqrt(smallc2), s.red_out, s.dt_dev,
+                                     s.t_dev, s.stream);
+    }
     if (dt_fixed <= 0.0) {
         hipError_t e = launch_courant(s.uold, s.refined, head_idx, num_octs, s.ndim, dx, gamma, smallr,
                                       __builtin_sqrt(smallc2), courant_factor, g ? g : gz, s.red_part, s.red_cap,
@@ -214,6 +249,15 @@ int rh_level_run(int head_idx, int num_octs, int ilevel, double dx, double gamma
                                      slope, riemann, dtp, dt_fixed, a1);
     if (r) return r;
     RH_HIP_CHECK(hipMemsetAsync(s.t_dev, 0, sizeof(double), s.stream));
+    if (s.ndim == 3 && dt_fixed <= 0.0) {
+        // dt of the first step from a standalone Courant pass; each step's
+        // kernel then produces the dt of the next one
+        if (int r2 = ensure_partials(lb->nbricks)) return r2;
+        const double gz[3] = {0, 0, 0};
+        RH_HIP_CHECK(launch_courant(s.uold, s.refined, head_idx, num_octs, s.ndim, dx, gamma, smallr,
+                                    __builtin_sqrt(smallc2), courant_factor, g ? g : gz, s.red_part, s.red_cap,
+                                    s.red_out, s.dt_dev, s.stream, nullptr));
+    }
     if (use_graph && nsteps >= 2) {
         // capture the pair of steps uold->unew, unew->uold as one graph
         hipGraph_t graph;

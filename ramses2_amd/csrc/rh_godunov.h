@@ -31,6 +31,15 @@ struct GodArgs {
     double g[3];
     int slope, riemann;
     int xcd_remap;            // remap block ids so each XCD gets contiguous bricks
+    // coarse-fine refluxing (levels above levelmin): octs > ngridmax are
+    // interpolated ghosts whose parent cell is ghost_parent[2*(v-ngridmax-1)+{0,1}]
+    int reflux;
+    int ngridmax;
+    const int32_t *ghost_parent;
+    // fused courant_fine of the updated state: per-brick partials
+    // (mass, ekin, eint, dt) at cpart[4*brick]
+    int courant;
+    double *cpart;
 };
 
 // 1D arguments: one thread per cell of the level, 3-slot neighbour table.
@@ -49,6 +58,9 @@ struct God1Args {
     int use_smax;
     double g;
     int slope, riemann;
+    int reflux;
+    int ngridmax;
+    const int32_t *ghost_parent;
 };
 
 // Launch the 3D brick kernel for brick shape (bx,by,bz) in precision prec
@@ -61,10 +73,38 @@ hipError_t launch_godunov3d_wl(const GodArgs *descs, const int2 *items, int nite
 
 // Courant reduction over cells of octs [head, head+noct): per-block partials,
 // then a fixed-order final pass -> out[0..4] = mass, ekin, eint, emag, dt.
+// Fixed-order reduction of fused-Courant partials: t_acc += the dt the step
+// used (*dt_inout on entry), then *dt_inout = min(dtmax, partial dts).
+hipError_t launch_courant_reduce(const double *partials, int nparts, double dtmax, double *out, double *dt_inout,
+                                 double *t_acc, hipStream_t s);
+
 hipError_t launch_courant(const double *uold, const uint8_t *refined, int head, int noct, int ndim,
                           double dx, double gamma, double smallr, double smallc, double courant_factor,
                           const double g[3], double *partials, int max_blocks, double *out,
                           double *dt_out, hipStream_t s, double *t_acc = nullptr);
+
+// Ghost octs of a level: fill each from its parent cell and the 2*ndim
+// neighbouring parent cells by interpol_hydro (hydro/interpol_hydro.f90:5-113).
+// src[g*7 + j] = (oct code, cell index 1-based) of u1(j); keys for boundary codes.
+struct GhostArgs {
+    double *uold;
+    const uint8_t *refined;
+    const int32_t *src;      // [nghost][7][2]
+    const int32_t *srckey;   // [nghost][7][3]
+    int first;               // oct index (1-based) of the first ghost of the level
+    int nghost;
+    int ndim;
+    int interpol_var, interpol_type;
+    double smallr;
+    int boxmin[3], boxmax[3];  // coarse level box
+    int btype[3][2];
+};
+hipError_t launch_refresh_ghosts(const GhostArgs &a, hipStream_t s);
+
+// Restriction (upload_fine, hydro/upload.f90:74-283): average each fine oct of
+// [head, head+noct) into its parent cell father[2*i], father[2*i+1].
+hipError_t launch_upload(double *uold, const int32_t *father, int head, int noct, int ndim, int internal_energy,
+                         double smallr, hipStream_t s);
 
 // Ordered brick-shape support list: returns true when (bx,by,bz) has a compiled kernel.
 bool brick_shape_supported(int bx, int by, int bz);

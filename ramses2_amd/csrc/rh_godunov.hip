@@ -19,6 +19,84 @@
 
 namespace rh {
 
+struct CourantK {
+    double dx, vol, gamma, smallr, smallc, courant_factor, smallp, gsum_dx;
+    int ndim;
+};
+
+// cmpdt, hydro/godunov_utils.f90:228-337, for one cell (no MHD).
+__device__ __forceinline__ double cell_dt(double uu0, double uu1, double uu2, double uu3, double uu4, const CourantK &c)
+{
+    double r = uu0 > c.smallr ? uu0 : c.smallr;
+    double v[3] = {uu1 / r, uu2 / r, uu3 / r};
+    double e = uu4;
+    e = e - 0.5 * r * (v[0] * v[0]);
+    e = e - 0.5 * r * (v[1] * v[1]);
+    e = e - 0.5 * r * (v[2] * v[2]);
+    double pe = (c.gamma - 1.0) * e, pf = r * c.smallp;
+    double p = pe > pf ? pe : pf;
+    double a2 = c.gamma * p;
+    a2 = a2 / r;
+    double c2 = 0.5 * (0.0 / r + a2);
+    double cf = __builtin_sqrt(c2 + __builtin_sqrt(c2 * c2 - 0.0));
+    double ctot = 0.0;
+    for (int d = 0; d < c.ndim; d++) ctot = ctot + __builtin_fabs(v[d]) + cf;
+    double grav = c.gsum_dx / (ctot * ctot);
+    grav = grav > 0.0001 ? grav : 0.0001;
+    double dtmax = c.courant_factor * c.dx / c.smallc;
+    double dtcell = 2.0 * c.courant_factor * c.dx / (ctot * (1.0 + __builtin_sqrt(1.0 + 2.0 * c.courant_factor * grav)));
+    return dtmax < dtcell ? dtmax : dtcell;
+}
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+__device__ __forceinline__ double wave_min(double x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double y = __shfl_xor(x, o, 64);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
+__device__ __forceinline__ CourantK make_courant_k(double dx, int ndim, double gamma, double smallr, double smallc2,
+                                                   double courant_factor, const double g[3])
+{
+    CourantK c;
+    c.dx = dx;
+    c.vol = ndim == 1 ? dx : dx * dx * dx;
+    c.gamma = gamma;
+    c.smallr = smallr;
+    c.smallc = __builtin_sqrt(smallc2);
+    c.courant_factor = courant_factor;
+    c.smallp = smallc2 / gamma;
+    double gs = 0.0;
+    for (int d = 0; d < ndim; d++) gs = gs + __builtin_fabs(g[d]);
+    c.gsum_dx = gs * dx;
+    c.ndim = ndim;
+    return c;
+}
+
+// One leaf cell of courant_fine (hydro/courant_fine.f90:120-160).
+__device__ __forceinline__ void courant_cell(const double u[5], const CourantK &c, double &mass, double &ekin,
+                                             double &eint, double &dtmin)
+{
+    mass = mass + u[0] * c.vol;
+    ekin = ekin + u[4] * c.vol;
+    eint = eint + u[4] * c.vol;
+    const double rr = u[0] > c.smallr ? u[0] : c.smallr;
+    eint = eint - 0.5 * (u[1] * u[1]) / rr * c.vol;
+    eint = eint - 0.5 * (u[2] * u[2]) / rr * c.vol;
+    eint = eint - 0.5 * (u[3] * u[3]) / rr * c.vol;
+    const double d = cell_dt(u[0], u[1], u[2], u[3], u[4], c);
+    dtmin = d < dtmin ? d : dtmin;
+}
+
 // Gather the conservative state of cell (i2,j2,k2) of the oct at window slot
 // value v, whose oct key is okey.  Boundary ghosts (v<0) follow
 // init_bound_refine (amr/boundaries.f90:39-165): zero gradient copies the
@@ -99,6 +177,7 @@ __device__ __forceinline__ HydroK<T> make_k(const double dt, double dx, double g
 // for variable n of the cell at ring index (s0, pc).
 template <typename T, int PX, int NP>
 __device__ __forceinline__ void slope27(const T (*sq)[5][NP], int sm, int s0, int sp, int n, int pc, T dq3[3])
+
 {
     const T qc = sq[s0][n][pc];
     T vmin = T(0), vmax = T(0);
@@ -129,20 +208,43 @@ __device__ __forceinline__ void slope27(const T (*sq)[5][NP], int sm, int s0, in
     dq3[2] = slop * dfz;
 }
 
-// LDS of one brick workgroup.
-template <int BX, int BY, int BZ, typename T> struct BrickSmem {
+// Coarse-fine refluxing, godfine1 (hydro/godunov_fine.f90:894-1052): a face
+// between a cell of the level and an interpolated ghost cell sends -flux/2^ndim
+// (ghost on the low side) or +flux/2^ndim (ghost on the high side) to the
+// ghost's parent cell.  FP64 atomics: the order of the contributions to one
+// parent cell is not fixed (last-bit differences from the serial reference).
+__device__ __forceinline__ void reflux_add(double *unew, const int32_t *ghost_parent, int ngridmax, int vghost,
+                                           double sign, const double F[5], int ttd)
+{
+    const int gi = vghost - ngridmax - 1;
+    const int igrid = ghost_parent[2 * gi], icell = ghost_parent[2 * gi + 1];
+    if (igrid <= 0) return;  // parent missing: rh_build_nbor reported it
+    const double w = 1.0 / (double)ttd;
+    double *p = unew + (size_t)(igrid - 1) * 5 * ttd + (icell - 1);
+#pragma unroll
+    for (int n = 0; n < 5; n++) atomicAdd(p + n * ttd, sign * (F[n] * w));
+}
+
+// LDS of one brick workgroup.  NS primitive planes are kept: 2 when the slopes
+// need only face neighbours (the z neighbours of a column sit in registers),
+// 3 for the 27-point slope of slope_type 3.
+template <int BX, int BY, int BZ, typename T, int NS> struct BrickSmem {
     static constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
     static constexpr int PX = 2 * BX + 4, PY = 2 * BY + 4, NP = PX * PY;
     static constexpr int WX = BX + 2, WY = BY + 2, WZ = BZ + 2, NW = WX * WY * WZ;
     int32_t swin[NW];
-    T sq[3][5][NP];
-    uint8_t sref[3][NP];
+    T sq[NS][5][NP];
+    uint8_t sref[NS][NP];
     T sf[2][5][NT];
 };
 
+__device__ __forceinline__ int ring_slot2(int t) { return t & 1; }
+__device__ __forceinline__ int ring_slot3(int t) { return (t + 3) % 3; }
+
 // One brick: the body shared by the per-level kernel and the work-list kernel.
-template <int BX, int BY, int BZ, typename T>
-__device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, BrickSmem<BX, BY, BZ, T> &sm)
+// RS = 3 compiles HLLC in; NS is the depth of the primitive ring.
+template <int BX, int BY, int BZ, typename T, int RS, int NS>
+__device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, BrickSmem<BX, BY, BZ, T, NS> &sm)
 {
     constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
     constexpr int PX = 2 * BX + 4, PY = 2 * BY + 4, NP = PX * PY;
@@ -176,9 +278,12 @@ __device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, Bric
     T qmz_prev[5] = {T(0), T(0), T(0), T(0), T(0)};
     uint8_t ref_prev = 0;
     double fz_prev[5] = {0, 0, 0, 0, 0}, part_prev[5] = {0, 0, 0, 0, 0};
+    // own column primitives of planes tt-1 and tt (used when NS == 2)
+    T qcol_m[5] = {T(0), T(0), T(0), T(0), T(0)}, qcol_0[5] = {T(0), T(0), T(0), T(0), T(0)};
+
 
     for (int t = -2; t <= 2 * BZ + 1; ++t) {
-        const int st = (t + 3) % 3;
+        const int st = NS == 3 ? ring_slot3(t) : ring_slot2(t);
         // ---- P: primitives of plane t ----
         for (int i = tid; i < NP; i += NT) {
             const int px = i % PX - 2, py = i / PX - 2;
@@ -195,18 +300,29 @@ __device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, Bric
             sref[st][i] = rf;
         }
         __syncthreads();
-        if (t < 0) continue;
+        T qcol_p[5];
+#pragma unroll
+        for (int n = 0; n < 5; n++) qcol_p[n] = sq[st][n][pc];
+        if (t < 0) {
+#pragma unroll
+            for (int n = 0; n < 5; n++) {
+                qcol_m[n] = qcol_0[n];
+                qcol_0[n] = qcol_p[n];
+            }
+            continue;
+        }
         // ---- T: slopes and predictor of plane tt = t-1 ----
         const int tt = t - 1;
-        const int sm = (t + 1) % 3, s0 = (t + 2) % 3, sp = st;
+        const int s0 = NS == 3 ? ring_slot3(t - 1) : ring_slot2(t - 1);
+        const int smm = NS == 3 ? ring_slot3(t - 2) : 0;
         T q0[5], dq[3][5];
 #pragma unroll
-        for (int n = 0; n < 5; n++) q0[n] = sq[s0][n][pc];
-        if (k.slope_type == 3) {
+        for (int n = 0; n < 5; n++) q0[n] = qcol_0[n];
+        if (NS == 3 && k.slope_type == 3) {
 #pragma unroll
             for (int n = 0; n < 5; n++) {
                 T d3[3];
-                slope27<T, PX, NP>(sq, sm, s0, sp, n, pc, d3);
+                slope27<T, PX, NP>(sq, smm, s0, st, n, pc, d3);
                 dq[0][n] = d3[0];
                 dq[1][n] = d3[1];
                 dq[2][n] = d3[2];
@@ -217,19 +333,25 @@ __device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, Bric
             for (int n = 0; n < 5; n++) {
                 dq[0][n] = sl ? slope1(sl, sq[s0][n][pc - 1], q0[n], sq[s0][n][pc + 1]) : T(0);
                 dq[1][n] = sl ? slope1(sl, sq[s0][n][pc - PX], q0[n], sq[s0][n][pc + PX]) : T(0);
-                dq[2][n] = sl ? slope1(sl, sq[sm][n][pc], q0[n], sq[sp][n][pc]) : T(0);
+                dq[2][n] = sl ? slope1(sl, qcol_m[n], q0[n], qcol_p[n]) : T(0);
             }
         }
+        const T inv_r0 = trcp(q0[0]);
         if (k.slope_type > 0) {
-            dq[0][1] = vel_bound(dq[0][1], sq[s0][0][pc - 1], sq[s0][1][pc - 1], q0[0], q0[1], sq[s0][0][pc + 1],
+            dq[0][1] = vel_bound(dq[0][1], sq[s0][0][pc - 1], sq[s0][1][pc - 1], inv_r0, q0[1], sq[s0][0][pc + 1],
                                  sq[s0][1][pc + 1]);
-            dq[1][2] = vel_bound(dq[1][2], sq[s0][0][pc - PX], sq[s0][2][pc - PX], q0[0], q0[2],
+            dq[1][2] = vel_bound(dq[1][2], sq[s0][0][pc - PX], sq[s0][2][pc - PX], inv_r0, q0[2],
                                  sq[s0][0][pc + PX], sq[s0][2][pc + PX]);
-            dq[2][3] = vel_bound(dq[2][3], sq[sm][0][pc], sq[sm][3][pc], q0[0], q0[3], sq[sp][0][pc], sq[sp][3][pc]);
+            dq[2][3] = vel_bound(dq[2][3], qcol_m[0], qcol_m[3], inv_r0, q0[3], qcol_p[0], qcol_p[3]);
         }
         T qp[3][5], qm[3][5];
-        trace3(q0, dq, qp, qm, k);
+        trace3(q0, dq, qp, qm, k, inv_r0);
         const uint8_t myref = sref[s0][pc];
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+            qcol_m[n] = qcol_0[n];
+            qcol_0[n] = qcol_p[n];
+        }
 #pragma unroll
         for (int n = 0; n < 5; n++) {
             sf[0][n][tid] = qm[0][n];
@@ -245,7 +367,7 @@ __device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, Bric
             T ql[5];
 #pragma unroll
             for (int n = 0; n < 5; n++) ql[n] = sf[0][n][tid - 1];
-            face_flux(0, ql, qp[0], fx, k);
+            face_flux<RS>(0, ql, qp[0], fx, k);
             if (myref || sref[s0][pc - 1])
 #pragma unroll
                 for (int n = 0; n < 5; n++) fx[n] = T(0);
@@ -254,17 +376,46 @@ __device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, Bric
             T ql[5];
 #pragma unroll
             for (int n = 0; n < 5; n++) ql[n] = sf[1][n][tid - CX];
-            face_flux(1, ql, qp[1], fy, k);
+            face_flux<RS>(1, ql, qp[1], fy, k);
             if (myref || sref[s0][pc - PX])
 #pragma unroll
                 for (int n = 0; n < 5; n++) fy[n] = T(0);
         }
         if (interior && tt >= 0 && tt <= 2 * BZ) {
             T f[5];
-            face_flux(2, qmz_prev, qp[2], f, k);
+            face_flux<RS>(2, qmz_prev, qp[2], f, k);
             const bool z = myref || ref_prev;
 #pragma unroll
             for (int n = 0; n < 5; n++) fz[n] = z ? 0.0 : (double(f[n]) * dt) * inv_dx;
+        }
+        if (a.reflux) {
+            // window value and ownership of a cell of the trace region
+            auto wv = [&](int x, int y, int z) { return swin[(((z + 2) >> 1) * WY + ((y + 2) >> 1)) * WX + ((x + 2) >> 1)]; };
+            auto owned = [&](int x, int y, int z, int v) {
+                return x >= 0 && x < 2 * BX && y >= 0 && y < 2 * BY && z >= 0 && z < 2 * BZ && v >= a.head &&
+                       v < a.head + a.noct && ox0 + (x >> 1) < boxmax[0] && oy0 + (y >> 1) < boxmax[1] &&
+                       oz0 + (z >> 1) < boxmax[2];
+            };
+            double F[5];
+            if (wfx && !(myref || sref[s0][pc - 1])) {
+                const int vl = wv(cx - 1, cy, tt), vh = wv(cx, cy, tt);
+#pragma unroll
+                for (int n = 0; n < 5; n++) F[n] = (double(fx[n]) * dt) * inv_dx;
+                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8);
+                if (vh > a.ngridmax && owned(cx - 1, cy, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8);
+            }
+            if (wfy && !(myref || sref[s0][pc - PX])) {
+                const int vl = wv(cx, cy - 1, tt), vh = wv(cx, cy, tt);
+#pragma unroll
+                for (int n = 0; n < 5; n++) F[n] = (double(fy[n]) * dt) * inv_dx;
+                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8);
+                if (vh > a.ngridmax && owned(cx, cy - 1, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8);
+            }
+            if (interior && tt >= 0 && tt <= 2 * BZ && !(myref || ref_prev)) {
+                const int vl = wv(cx, cy, tt - 1), vh = wv(cx, cy, tt);
+                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, fz, 8);
+                if (vh > a.ngridmax && owned(cx, cy, tt - 1, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, fz, 8);
+            }
         }
         __syncthreads();
         if (wfx)
@@ -314,12 +465,64 @@ __device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, Bric
         for (int n = 0; n < 5; n++) qmz_prev[n] = qm[2][n];
         ref_prev = myref;
     }
+    if (a.courant) {
+        // courant_fine of the state just written (the dt of the next step):
+        // the brick's own leaf cells, re-read from unew while still in cache,
+        // then the brick's partials in a fixed reduction order (FP64)
+        const CourantK ck = make_courant_k(a.dx, 3, a.gamma, a.smallr, a.smallc2, a.courant_factor, a.g);
+        double c_mass = 0.0, c_ekin = 0.0, c_eint = 0.0, c_dt = a.dx / ck.smallc;
+        __syncthreads();
+        for (int i = tid; i < BX * BY * BZ * 8; i += NT) {
+            const int oc = i >> 3, ind = i & 7;
+            const int ox = oc % BX + 1, oy = (oc / BX) % BY + 1, oz = oc / (BX * BY) + 1;
+            const int v = swin[(oz * WY + oy) * WX + ox];
+            const bool own = v >= a.head && v < a.head + a.noct && ox0 + ox - 1 < boxmax[0] &&
+                             oy0 + oy - 1 < boxmax[1] && oz0 + oz - 1 < boxmax[2];
+            if (!own || a.refined[(size_t)(v - 1) * 8 + ind]) continue;
+            const double *pu = a.unew + (size_t)(v - 1) * 40 + ind;
+            const double un[5] = {pu[0], pu[8], pu[16], pu[24], pu[32]};
+            courant_cell(un, ck, c_mass, c_ekin, c_eint, c_dt);
+        }
+        c_mass = wave_sum(c_mass);
+        c_ekin = wave_sum(c_ekin);
+        c_eint = wave_sum(c_eint);
+        c_dt = wave_min(c_dt);
+        __syncthreads();
+        double *red = (double *)&sf[0][0][0];
+        const int w = tid >> 6;
+        if ((tid & 63) == 0) {
+            red[4 * w + 0] = c_mass;
+            red[4 * w + 1] = c_ekin;
+            red[4 * w + 2] = c_eint;
+            red[4 * w + 3] = c_dt;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double m = 0, e = 0, ei = 0, d = red[3];
+            for (int i = 0; i < (NT + 63) / 64; i++) {
+                m += red[4 * i];
+                e += red[4 * i + 1];
+                ei += red[4 * i + 2];
+                d = red[4 * i + 3] < d ? red[4 * i + 3] : d;
+            }
+            a.cpart[4 * bid + 0] = m;
+            a.cpart[4 * bid + 1] = e;
+            a.cpart[4 * bid + 2] = ei;
+            a.cpart[4 * bid + 3] = d;
+        }
+    }
 }
 
-template <int BX, int BY, int BZ, typename T>
-__global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2)) k_godunov3d(const GodArgs a)
+// Minimum waves per SIMD requested from the register allocator for the brick
+// kernels (occupancy vs. register spills; chosen by measurement).
+#ifndef RH_BRICK_MIN_WAVES
+#define RH_BRICK_MIN_WAVES 2
+#endif
+
+template <int BX, int BY, int BZ, typename T, int RS, int NS>
+__global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), RH_BRICK_MIN_WAVES) k_godunov3d(const GodArgs a)
 {
-    __shared__ BrickSmem<BX, BY, BZ, T> sm;
+    __shared__ BrickSmem<BX, BY, BZ, T, NS> sm;
     // XCD-aware brick assignment: consecutive bricks to the same XCD so that
     // their shared halo octs meet in that XCD's L2.
     int bid = blockIdx.x;
@@ -329,20 +532,20 @@ __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2)) k_godunov3d(const
         bid = x * q + (x < r ? x : r) + l;
     }
     if (bid >= a.nbricks) return;
-    brick_body<BX, BY, BZ, T>(a, bid, sm);
+    brick_body<BX, BY, BZ, T, RS, NS>(a, bid, sm);
 }
 
 // Persistent work-list kernel: work item i = (descriptor, brick).  Descriptors
 // carry the arrays and parameters of one level of one job, so bricks of
 // several levels and of several jobs share one launch.
-template <int BX, int BY, int BZ, typename T>
+template <int BX, int BY, int BZ, typename T, int RS, int NS>
 __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2))
     k_godunov3d_wl(const GodArgs *__restrict__ descs, const int2 *__restrict__ items, int nitems)
 {
-    __shared__ BrickSmem<BX, BY, BZ, T> sm;
+    __shared__ BrickSmem<BX, BY, BZ, T, NS> sm;
     for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
         const int2 it = items[i];
-        brick_body<BX, BY, BZ, T>(descs[it.x], it.y, sm);
+        brick_body<BX, BY, BZ, T, RS, NS>(descs[it.x], it.y, sm);
         __syncthreads();
     }
 }
@@ -387,8 +590,9 @@ __global__ void __launch_bounds__(256) k_godunov1d(const God1Args a)
         T dq[5];
 #pragma unroll
         for (int n = 0; n < 5; n++) dq[n] = k.slope_type ? slope1(k.slope_type, q[c - 1][n], q[c][n], q[c + 1][n]) : T(0);
-        if (k.slope_type > 0) dq[1] = vel_bound(dq[1], q[c - 1][0], q[c - 1][1], q[c][0], q[c][1], q[c + 1][0], q[c + 1][1]);
-        trace1(q[c], dq, qp[c - 1], qm[c - 1], k);
+        const T inv_r = trcp(q[c][0]);
+        if (k.slope_type > 0) dq[1] = vel_bound(dq[1], q[c - 1][0], q[c - 1][1], inv_r, q[c][1], q[c + 1][0], q[c + 1][1]);
+        trace1(q[c], dq, qp[c - 1], qm[c - 1], k, inv_r);
     }
     T fl[5], fr[5];
     face_flux(0, qm[0], qp[1], fl, k);
@@ -397,62 +601,24 @@ __global__ void __launch_bounds__(256) k_godunov1d(const God1Args a)
     const int ind = i2;
     const double *pb = a.ubase + (size_t)(o - 1) * 10 + ind;
     double *po = a.unew + (size_t)(o - 1) * 10 + ind;
+    double FLs[5], FRs[5];
 #pragma unroll
     for (int n = 0; n < 5; n++) {
-        const double FL = zl ? 0.0 : (double(fl[n]) * dt) * a.inv_dx;
-        const double FR = zr ? 0.0 : (double(fr[n]) * dt) * a.inv_dx;
-        po[2 * n] = pb[2 * n] + (FL - FR);
+        FLs[n] = zl ? 0.0 : (double(fl[n]) * dt) * a.inv_dx;
+        FRs[n] = zr ? 0.0 : (double(fr[n]) * dt) * a.inv_dx;
+        po[2 * n] = pb[2 * n] + (FLs[n] - FRs[n]);
+    }
+    if (a.reflux) {
+        // the left neighbour cell lies in this oct (i2=1) or in slot 0; the right one in this oct or slot 2
+        const int vl = i2 == 1 ? o : nb[0], vr = i2 == 0 ? o : nb[2];
+        if (!zl && vl > a.ngridmax) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, FLs, 2);
+        if (!zr && vr > a.ngridmax) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vr, 1.0, FRs, 2);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Courant time step and conservation sums (courant_fine + cmpdt).
 // ---------------------------------------------------------------------------
-struct CourantK {
-    double dx, vol, gamma, smallr, smallc, courant_factor, smallp, gsum_dx;
-    int ndim;
-};
-
-// cmpdt, hydro/godunov_utils.f90:228-337, for one cell (no MHD).
-__device__ __forceinline__ double cell_dt(double uu0, double uu1, double uu2, double uu3, double uu4, const CourantK &c)
-{
-    double r = uu0 > c.smallr ? uu0 : c.smallr;
-    double v[3] = {uu1 / r, uu2 / r, uu3 / r};
-    double e = uu4;
-    e = e - 0.5 * r * (v[0] * v[0]);
-    e = e - 0.5 * r * (v[1] * v[1]);
-    e = e - 0.5 * r * (v[2] * v[2]);
-    double pe = (c.gamma - 1.0) * e, pf = r * c.smallp;
-    double p = pe > pf ? pe : pf;
-    double a2 = c.gamma * p;
-    a2 = a2 / r;
-    double c2 = 0.5 * (0.0 / r + a2);
-    double cf = __builtin_sqrt(c2 + __builtin_sqrt(c2 * c2 - 0.0));
-    double ctot = 0.0;
-    for (int d = 0; d < c.ndim; d++) ctot = ctot + __builtin_fabs(v[d]) + cf;
-    double grav = c.gsum_dx / (ctot * ctot);
-    grav = grav > 0.0001 ? grav : 0.0001;
-    double dtmax = c.courant_factor * c.dx / c.smallc;
-    double dtcell = 2.0 * c.courant_factor * c.dx / (ctot * (1.0 + __builtin_sqrt(1.0 + 2.0 * c.courant_factor * grav)));
-    return dtmax < dtcell ? dtmax : dtcell;
-}
-
-__device__ __forceinline__ double wave_sum(double x)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;
-}
-__device__ __forceinline__ double wave_min(double x)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        double y = __shfl_xor(x, o, 64);
-        x = y < x ? y : x;
-    }
-    return x;
-}
-
 constexpr int kCourantThreads = 256;
 
 __global__ void __launch_bounds__(kCourantThreads) k_courant(const double *__restrict__ uold,
@@ -467,16 +633,8 @@ __global__ void __launch_bounds__(kCourantThreads) k_courant(const double *__res
         const int ind = (int)(i % ttd);
         if (refined[o * 8 + ind]) continue;
         const double *p = uold + o * stride + ind;
-        const double u0 = p[0], u1 = p[ttd], u2 = p[2 * ttd], u3 = p[3 * ttd], u4 = p[4 * ttd];
-        mass = mass + u0 * c.vol;
-        ekin = ekin + u4 * c.vol;
-        eint = eint + u4 * c.vol;
-        const double rr = u0 > c.smallr ? u0 : c.smallr;
-        eint = eint - 0.5 * (u1 * u1) / rr * c.vol;
-        eint = eint - 0.5 * (u2 * u2) / rr * c.vol;
-        eint = eint - 0.5 * (u3 * u3) / rr * c.vol;
-        const double d = cell_dt(u0, u1, u2, u3, u4, c);
-        dtmin = d < dtmin ? d : dtmin;
+        const double u[5] = {p[0], p[ttd], p[2 * ttd], p[3 * ttd], p[4 * ttd]};
+        courant_cell(u, c, mass, ekin, eint, dtmin);
     }
     __shared__ double red[kCourantThreads / 64][4];
     mass = wave_sum(mass);
@@ -510,7 +668,7 @@ __global__ void __launch_bounds__(kCourantThreads) k_courant(const double *__res
 // emag, dt}; dt_out (optional) receives dt for the next Godunov launch.
 __global__ void __launch_bounds__(256) k_courant_final(const double *__restrict__ part, int nblocks, double dtmax,
                                                        double *__restrict__ out, double *__restrict__ dt_out,
-                                                       double *__restrict__ t_acc)
+                                                       double *__restrict__ t_acc, int acc_old)
 {
     __shared__ double red[4][4];
     double s0 = 0, s1 = 0, s2 = 0, m = dtmax;
@@ -545,9 +703,149 @@ __global__ void __launch_bounds__(256) k_courant_final(const double *__restrict_
         out[2] = a2;
         out[3] = 0.0;
         out[4] = mm;
+        if (t_acc) *t_acc += (acc_old && dt_out) ? *dt_out : mm;
         if (dt_out) *dt_out = mm;
-        if (t_acc) *t_acc += mm;
     }
+}
+
+// ---------------------------------------------------------------------------
+// AMR: ghost octs (interpolated from the parent level) and restriction
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void cell_xc(int ndim, int ind, double xc[3])
+{
+    const int iz = (ind - 1) / 4, iy = (ind - 1 - 4 * iz) / 2, ix = ind - 1 - 2 * iy - 4 * iz;
+    xc[0] = (double)ix - 0.5;
+    xc[1] = ndim > 1 ? (double)iy - 0.5 : 0.0;
+    xc[2] = ndim > 2 ? (double)iz - 0.5 : 0.0;
+}
+
+// interpol_hydro (hydro/interpol_hydro.f90:5-113) with the limiters of
+// :610-752; u1[j][ivar], j = 0 (central parent) and 1..2ndim (-x,+x,-y,+y,-z,+z).
+__device__ void interpol_hydro_dev(int ndim, double u1[7][5], double u2[8][5], int interpol_var, int interpol_type,
+                                   double smallr)
+{
+    const int twondim = 2 * ndim, ttd = 1 << ndim;
+    if (interpol_var == 1)
+        for (int j = 0; j <= twondim; j++) {
+            double ekin = 0.0;
+            for (int d = 1; d <= 3; d++) ekin = ekin + 0.5 * (u1[j][d] * u1[j][d]) / fmax(u1[j][0], smallr);
+            u1[j][4] = u1[j][4] - ekin - 0.0;
+        }
+    for (int iv = 0; iv < 5; iv++) {
+        double a[7], w[3] = {0, 0, 0};
+        for (int j = 0; j <= twondim; j++) a[j] = u1[j][iv];
+        if (interpol_type == 1) {
+            for (int d = 1; d <= ndim; d++) {
+                const double dl = 0.5 * (a[2 * d] - a[0]), dr = 0.5 * (a[0] - a[2 * d - 1]);
+                w[d - 1] = (dl * dr <= 0.0) ? 0.0 : fmin(fabs(dl), fabs(dr)) * dl / fabs(dl);
+            }
+        } else if (interpol_type == 2 || interpol_type == 3) {
+            for (int d = 1; d <= ndim; d++) w[d - 1] = 0.25 * (a[2 * d] - a[2 * d - 1]);
+            if (interpol_type == 2) {
+                double ac[8];
+                for (int c = 0; c < ttd; c++) ac[c] = a[0];
+                for (int d = 1; d <= ndim; d++)
+                    for (int c = 1; c <= ttd; c++) {
+                        double xc[3];
+                        cell_xc(ndim, c, xc);
+                        ac[c - 1] = ac[c - 1] + 2.0 * w[d - 1] * xc[d - 1];
+                    }
+                double cmax = ac[0], cmin = ac[0], kmax = a[1], kmin = a[1];
+                for (int c = 1; c < ttd; c++) {
+                    cmax = fmax(cmax, ac[c]);
+                    cmin = fmin(cmin, ac[c]);
+                }
+                for (int j = 2; j <= twondim; j++) {
+                    kmax = fmax(kmax, a[j]);
+                    kmin = fmin(kmin, a[j]);
+                }
+                double dk = a[0] - kmax, dc = a[0] - cmax;
+                const double lmax = (dk * dc > 0.0) ? fmin(1.0, dk / dc) : 0.0;
+                dk = a[0] - kmin;
+                dc = a[0] - cmin;
+                const double lmin = (dk * dc > 0.0) ? fmin(1.0, dk / dc) : 0.0;
+                const double lim = fmin(lmin, lmax);
+                for (int d = 0; d < ndim; d++) w[d] = w[d] * lim;
+            }
+        }
+        for (int c = 1; c <= ttd; c++) {
+            double xc[3];
+            cell_xc(ndim, c, xc);
+            double v = a[0];
+            for (int d = 1; d <= ndim; d++) v = v + w[d - 1] * xc[d - 1];
+            u2[c - 1][iv] = v;
+        }
+    }
+    if (interpol_var == 1)
+        for (int c = 0; c < ttd; c++) {
+            double ekin = 0.0;
+            for (int d = 1; d <= 3; d++) ekin = ekin + 0.5 * (u2[c][d] * u2[c][d]) / fmax(u2[c][0], smallr);
+            u2[c][4] = u2[c][4] + ekin + 0.0;
+        }
+}
+
+__global__ void __launch_bounds__(128) k_refresh_ghosts(const GhostArgs a)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.nghost) return;
+    const int ttd = 1 << a.ndim, nslot = 2 * a.ndim + 1;
+    const int stride = ttd * 5;
+    double u1[7][5], u2[8][5];
+    for (int j = 0; j < nslot; j++) {
+        const int code = a.src[(g * 7 + j) * 2], ind = a.src[(g * 7 + j) * 2 + 1];
+        const int okey[3] = {a.srckey[(g * 7 + j) * 3], a.srckey[(g * 7 + j) * 3 + 1], a.srckey[(g * 7 + j) * 3 + 2]};
+        int c[3] = {(ind - 1) & 1, ((ind - 1) >> 1) & 1, ((ind - 1) >> 2) & 1};
+        uint8_t rf;
+        gather_cell(a.uold, a.refined, code, okey, a.ndim, a.boxmin, a.boxmax, a.btype, c, u1[j], rf);
+    }
+    interpol_hydro_dev(a.ndim, u1, u2, a.interpol_var, a.interpol_type, a.smallr);
+    double *p = a.uold + (size_t)(a.first - 1 + g) * stride;
+    for (int c = 0; c < ttd; c++)
+        for (int n = 0; n < 5; n++) p[n * ttd + c] = u2[c][n];
+}
+
+hipError_t launch_refresh_ghosts(const GhostArgs &a, hipStream_t s)
+{
+    if (a.nghost <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_refresh_ghosts, dim3((a.nghost + 127) / 128), dim3(128), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_upload(double *uold, const int32_t *father, int head, int noct, int ttd,
+                                                int internal_energy, double smallr)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= noct) return;
+    const int o = head - 1 + i;
+    const int igrid = father[2 * i], icell = father[2 * i + 1];
+    if (igrid <= 0) return;
+    const double *c = uold + (size_t)o * 5 * ttd;
+    double *p = uold + (size_t)(igrid - 1) * 5 * ttd + (icell - 1);
+    for (int n = 0; n < 5; n++) {
+        double avg = 0.0;
+        for (int k = 0; k < ttd; k++) avg = avg + c[n * ttd + k];
+        p[n * ttd] = avg / (double)ttd;
+    }
+    if (internal_energy) {
+        double avg = 0.0;
+        for (int k = 0; k < ttd; k++) {
+            double ekin = 0.0;
+            for (int d = 1; d <= 3; d++) ekin = ekin + 0.5 * (c[d * ttd + k] * c[d * ttd + k]) / fmax(c[k], smallr);
+            avg = avg + c[4 * ttd + k] - ekin - 0.0 - 0.0;
+        }
+        double ekin = 0.0;
+        for (int d = 1; d <= 3; d++) ekin = ekin + 0.5 * (p[d * ttd] * p[d * ttd]) / fmax(p[0], smallr);
+        p[4 * ttd] = avg / (double)ttd + ekin + 0.0 + 0.0;
+    }
+}
+
+hipError_t launch_upload(double *uold, const int32_t *father, int head, int noct, int ndim, int internal_energy,
+                         double smallr, hipStream_t s)
+{
+    if (noct <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_upload, dim3((noct + 255) / 256), dim3(256), 0, s, uold, father, head, noct, 1 << ndim,
+                       internal_energy, smallr);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -558,9 +856,7 @@ __global__ void __launch_bounds__(256) k_courant_final(const double *__restrict_
     X(7, 7, 4)             \
     X(3, 3, 4)             \
     X(3, 3, 8)             \
-    X(3, 7, 8)             \
-    X(1, 1, 1)             \
-    X(1, 1, 2)
+    X(1, 1, 1)
 
 bool brick_shape_supported(int bx, int by, int bz)
 {
@@ -574,14 +870,27 @@ bool brick_shape_supported(int bx, int by, int bz)
 hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, hipStream_t s)
 {
     if (a.nbricks <= 0) return hipSuccess;
-#define RH_LAUNCH_SHAPE(A, B, C)                                                                      \
-    if (bx == A && by == B && bz == C) {                                                              \
-        constexpr int nt = (2 * A + 2) * (2 * B + 2);                                                  \
-        if (prec == 0)                                                                                \
-            hipLaunchKernelGGL((k_godunov3d<A, B, C, double>), dim3(a.nbricks), dim3(nt), 0, s, a);   \
-        else                                                                                          \
-            hipLaunchKernelGGL((k_godunov3d<A, B, C, float>), dim3(a.nbricks), dim3(nt), 0, s, a);    \
-        return hipGetLastError();                                                                     \
+#define RH_LAUNCH_SHAPE(A, B, C)                                                                         \
+    if (bx == A && by == B && bz == C) {                                                                 \
+        constexpr int nt = (2 * A + 2) * (2 * B + 2);                                                     \
+        const dim3 g(a.nbricks), b(nt);                                                                  \
+        if (a.slope == 3) {                                                                              \
+            if (prec == 0)                                                                               \
+                hipLaunchKernelGGL((k_godunov3d<A, B, C, double, 0, 3>), g, b, 0, s, a);                 \
+            else                                                                                         \
+                hipLaunchKernelGGL((k_godunov3d<A, B, C, float, 0, 3>), g, b, 0, s, a);                  \
+        } else if (a.riemann == 3) {                                                                     \
+            if (prec == 0)                                                                               \
+                hipLaunchKernelGGL((k_godunov3d<A, B, C, double, 3, 2>), g, b, 0, s, a);                 \
+            else                                                                                         \
+                hipLaunchKernelGGL((k_godunov3d<A, B, C, float, 3, 2>), g, b, 0, s, a);                  \
+        } else {                                                                                         \
+            if (prec == 0)                                                                               \
+                hipLaunchKernelGGL((k_godunov3d<A, B, C, double, 0, 2>), g, b, 0, s, a);                 \
+            else                                                                                         \
+                hipLaunchKernelGGL((k_godunov3d<A, B, C, float, 0, 2>), g, b, 0, s, a);                  \
+        }                                                                                                \
+        return hipGetLastError();                                                                        \
     }
     RH_BRICK_SHAPES(RH_LAUNCH_SHAPE)
 #undef RH_LAUNCH_SHAPE
@@ -592,16 +901,16 @@ hipError_t launch_godunov3d_wl(const GodArgs *descs, const int2 *items, int nite
                                int bz, int prec, hipStream_t s)
 {
     if (nitems <= 0) return hipSuccess;
-#define RH_LAUNCH_WL(A, B, C)                                                                                   \
-    if (bx == A && by == B && bz == C) {                                                                        \
-        constexpr int nt = (2 * A + 2) * (2 * B + 2);                                                            \
-        if (prec == 0)                                                                                          \
-            hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, double>), dim3(nblocks), dim3(nt), 0, s, descs, items,  \
-                               nitems);                                                                         \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, float>), dim3(nblocks), dim3(nt), 0, s, descs, items,   \
-                               nitems);                                                                         \
-        return hipGetLastError();                                                                               \
+#define RH_LAUNCH_WL(A, B, C)                                                                            \
+    if (bx == A && by == B && bz == C) {                                                                 \
+        constexpr int nt = (2 * A + 2) * (2 * B + 2);                                                     \
+        if (prec == 0)                                                                                   \
+            hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, double, 0, 3>), dim3(nblocks), dim3(nt), 0, s,   \
+                               descs, items, nitems);                                                    \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, float, 0, 3>), dim3(nblocks), dim3(nt), 0, s,    \
+                               descs, items, nitems);                                                    \
+        return hipGetLastError();                                                                        \
     }
     RH_BRICK_SHAPES(RH_LAUNCH_WL)
 #undef RH_LAUNCH_WL
@@ -617,6 +926,13 @@ hipError_t launch_godunov1d(const God1Args &a, int prec, hipStream_t s)
         hipLaunchKernelGGL((k_godunov1d<double>), dim3(nb), dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL((k_godunov1d<float>), dim3(nb), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_courant_reduce(const double *partials, int nparts, double dtmax, double *out, double *dt_inout,
+                                 double *t_acc, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(256), 0, s, partials, nparts, dtmax, out, dt_inout, t_acc, 1);
     return hipGetLastError();
 }
 
@@ -645,7 +961,7 @@ hipError_t launch_courant(const double *uold, const uint8_t *refined, int head, 
     hipLaunchKernelGGL(k_courant, dim3(nb), dim3(kCourantThreads), 0, s, uold, refined, head, noct, ttd, c, partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(256), 0, s, partials, nb, dx / smallc, out, dt_out, t_acc);
+    hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(256), 0, s, partials, nb, dx / smallc, out, dt_out, t_acc, 0);
     return hipGetLastError();
 }
 

@@ -13,8 +13,12 @@
 namespace rh {
 
 // Scalar helpers -----------------------------------------------------------------
-template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
-template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+// max/min as single v_max/v_min instructions (the reference's MAX/MIN; the two
+// differ only in the sign of a zero result and for NaN inputs)
+__device__ __forceinline__ double tmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ double tmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float tmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ float tmin(float a, float b) { return __builtin_fminf(a, b); }
 __device__ __forceinline__ double tcopysign(double a, double b) { return __builtin_copysign(a, b); }
 __device__ __forceinline__ float tcopysign(float a, float b) { return __builtin_copysignf(a, b); }
 __device__ __forceinline__ double fabs_(double a) { return __builtin_fabs(a); }
@@ -25,6 +29,10 @@ __device__ __forceinline__ float fabs_(float a) { return __builtin_fabsf(a); }
 __device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float tdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double tsqrt(double a) { return __builtin_sqrt(a); }
+// reciprocal used for a denominator shared by several quotients: a/b is then
+// formed as a*(1/b), within an ulp of the reference's separate divisions
+__device__ __forceinline__ double trcp(double b) { return 1.0 / b; }
+__device__ __forceinline__ float trcp(float b) { return __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float tsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 
 // Constants of one Godunov launch (hydro_params_t + derived quantities).
@@ -85,11 +93,11 @@ template <typename T> __device__ __forceinline__ T slope1(int st, T qm1, T q0, T
 
 // Normal-velocity slope bound, hydro/umuscl.f90:2361-2385.
 template <typename T>
-__device__ __forceinline__ T vel_bound(T dq, T rl, T ul, T r0, T u0, T rr, T ur)
+__device__ __forceinline__ T vel_bound(T dq, T rl, T ul, T inv_r0, T u0, T rr, T ur)
 {
     T dlft = rl * fabs_(u0 - ul);
     T drgt = rr * fabs_(ur - u0);
-    T dlim = tdiv(T(2) * tmin(dlft, drgt), r0);
+    T dlim = T(2) * tmin(dlft, drgt) * inv_r0;
     return tcopysign(tmin(fabs_(dq), dlim), dq);
 }
 
@@ -97,7 +105,8 @@ __device__ __forceinline__ T vel_bound(T dq, T rl, T ul, T r0, T u0, T rr, T ur)
 // q: cell primitives; dq[d][n]: slopes.  Outputs qp[d][n] (right state at the
 // left interface) and qm[d][n] (left state at the right interface).
 template <typename T>
-__device__ __forceinline__ void trace3(const T q[5], const T dq[3][5], T qp[3][5], T qm[3][5], const HydroK<T> &k)
+__device__ __forceinline__ void trace3(const T q[5], const T dq[3][5], T qp[3][5], T qm[3][5], const HydroK<T> &k,
+                                       T inv_r)
 {
     const T half = T(0.5);
     T r = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
@@ -106,9 +115,9 @@ __device__ __forceinline__ void trace3(const T q[5], const T dq[3][5], T qp[3][5
     T drz = half * dq[2][0], duz = half * dq[2][1], dvz = half * dq[2][2], dwz = half * dq[2][3], dpz = half * dq[2][4];
     T divv = dux + dvy + dwz;
     T sr0 = -u * drx - v * dry - w * drz - divv * r;
-    T su0 = -u * dux - v * duy - w * duz - tdiv(dpx, r);
-    T sv0 = -u * dvx - v * dvy - w * dvz - tdiv(dpy, r);
-    T sw0 = -u * dwx - v * dwy - w * dwz - tdiv(dpz, r);
+    T su0 = -u * dux - v * duy - w * duz - dpx * inv_r;
+    T sv0 = -u * dvx - v * dvy - w * dvz - dpy * inv_r;
+    T sw0 = -u * dwx - v * dwy - w * dwz - dpz * inv_r;
     T sp0 = -u * dpx - v * dpy - w * dpz - divv * k.gamma * p;
     T ws[5];
     ws[0] = r + sr0 * k.dtdx;
@@ -133,14 +142,14 @@ __device__ __forceinline__ void trace3(const T q[5], const T dq[3][5], T qp[3][5
 
 // trace1d, hydro/umuscl.f90:297-481.
 template <typename T>
-__device__ __forceinline__ void trace1(const T q[5], const T dq[5], T qp[5], T qm[5], const HydroK<T> &k)
+__device__ __forceinline__ void trace1(const T q[5], const T dq[5], T qp[5], T qm[5], const HydroK<T> &k, T inv_r)
 {
     const T half = T(0.5);
     T r = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
     T drx = half * dq[0], dux = half * dq[1], dvx = half * dq[2], dwx = half * dq[3], dpx = half * dq[4];
     T sr0 = -u * drx - dux * r;
     T sp0 = -u * dpx - dux * k.gamma * p;
-    T su0 = -u * dux - tdiv(dpx, r);
+    T su0 = -u * dux - dpx * inv_r;
     T sv0 = -u * dvx;
     T sw0 = -u * dwx;
     T ws[5];
@@ -185,14 +194,15 @@ __device__ __forceinline__ void riemann_hllc(const T ql[5], const T qr[5], T f[5
     T SR = tmax(ul, ur) + cmax;
     T rcl = rl * (ul - SL);
     T rcr = rr * (SR - ur);
-    T den = rcr + rcl;
-    T ustar = tdiv(rcr * ur + rcl * ul + (Pl - Pr), den);
-    T Ptotstar = tdiv(rcr * Pl + rcl * Pr + rcl * rcr * (ul - ur), den);
+    T inv_den = trcp(rcr + rcl);
+    T ustar = (rcr * ur + rcl * ul + (Pl - Pr)) * inv_den;
+    T Ptotstar = (rcr * Pl + rcl * Pr + rcl * rcr * (ul - ur)) * inv_den;
     bool lstar = ustar > T(0);
     T S = lstar ? SL : SR;
     T rs = lstar ? rl : rr, us = lstar ? ul : ur, Ps = lstar ? Pl : Pr, es = lstar ? etotl : etotr;
-    T rstar = tdiv(rs * (S - us), S - ustar);
-    T etotstar = tdiv((S - us) * es - Ps * us + Ptotstar * ustar, S - ustar);
+    T inv_sd = trcp(S - ustar);
+    T rstar = rs * (S - us) * inv_sd;
+    T etotstar = ((S - us) * es - Ps * us + Ptotstar * ustar) * inv_sd;
     T ro, uo, vo, wo, Po, eo;
     if (SL > T(0)) {
         ro = rl; uo = ul; vo = vl; wo = wl; Po = Pl; eo = etotl;
@@ -282,7 +292,8 @@ __device__ __forceinline__ void riemann_hll(const T ql[5], const T qr[5], T f[5]
 
 // cmpflxm, hydro/umuscl.f90:1472-1607: rotate to the face normal idim, solve,
 // rotate back.  ql/qr are (rho,u,v,w,p) in grid axes.
-template <typename T>
+// RS: 3 = HLLC at compile time, 0 = the solver named by k.riemann.
+template <int RS = 0, typename T>
 __device__ __forceinline__ void face_flux(int idim, const T ql[5], const T qr[5], T f[5], const HydroK<T> &k)
 {
     const int ln = idim + 1;
@@ -291,7 +302,7 @@ __device__ __forceinline__ void face_flux(int idim, const T ql[5], const T qr[5]
     T a[5] = {ql[0], ql[ln], ql[lt1], ql[lt2], ql[4]};
     T b[5] = {qr[0], qr[ln], qr[lt1], qr[lt2], qr[4]};
     T g[5];
-    if (k.riemann == 3)
+    if (RS == 3 || k.riemann == 3)
         riemann_hllc(a, b, g, k);
     else if (k.riemann == 1)
         riemann_llf(a, b, g, k);

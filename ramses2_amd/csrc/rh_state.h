@@ -36,11 +36,16 @@ struct LevelBricks {
     int boxmax[3] = {0, 0, 0};
 };
 
-// Neighbour table of one level: 3^ndim slots per oct of the level.
+// Neighbour table of one level: 3^ndim slots per oct of the level, the
+// parent cell of each oct (father, for restriction), and the level's ghost
+// octs (missing neighbours interpolated from the parent level).
 struct LevelNbor {
     int ilevel = 0;
     int head = 0, noct = 0;
-    int32_t *nbor = nullptr;  // [noct][3^ndim]
+    int32_t *nbor = nullptr;    // [noct][3^ndim]
+    int32_t *father = nullptr;  // [noct][2] (igrid, icell) at ilevel-1, or 0
+    int ghost_first = 0;        // first ghost oct index (1-based) of the level
+    int nghost = 0;
     int boxmin[3] = {0, 0, 0};
     int boxmax[3] = {0, 0, 0};
 };
@@ -78,6 +83,14 @@ struct State {
     // single-level device time stepping: dt and accumulated time on the device
     double *dt_dev = nullptr;
     double *t_dev = nullptr;
+    // ghost octs (interpolated missing neighbours), indices ngridmax+1..ngridmax+nghost_total
+    int nghost_total = 0;
+    int32_t *ghost_parent = nullptr;  // [ncachemax][2]
+    int32_t *ghost_src = nullptr;     // [ncachemax][7][2]
+    int32_t *ghost_srckey = nullptr;  // [ncachemax][7][3]
+    int32_t *ghost_key = nullptr;     // [ncachemax][3]
+    int32_t *d_counter = nullptr;     // device scratch counters [4]
+    int interpol_var = 0, interpol_type = 1;
     // reduction scratch
     double *red_part = nullptr;
     int red_cap = 0;

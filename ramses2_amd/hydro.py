@@ -225,3 +225,61 @@ class UniformLevel:
 
     def cells(self):
         return self.mesh.to_cells(self.state())
+
+
+class AmrHierarchy:
+    """A multi-level mesh (AmrMesh) on the device, stepped in the order of
+    amr_step with nsubcycle=1 on every level (amr/amr_step.f90:276-588):
+    newdt/courant and set_unew going down the levels, then, finest first,
+    godunov_fine (refluxing into the coarser unew), set_uold and upload_fine.
+    All levels share dt = min over levels (amr_step.f90:588)."""
+
+    def __init__(self, amesh, uold, params: HydroParams, ncachemax=None, interpol_var=0, interpol_type=1,
+                 precision=0):
+        self.am = amesh
+        self.p = params
+        self.precision = precision
+        nc = ncachemax if ncachemax is not None else max(4 * amesh.ngrid, 10000)
+        self.hh = HipHydro(3, amesh.ngrid, nc)
+        ckey_max, key_off, box_min, box_max = amesh_level_params(amesh)
+        self.hh.upload_level_params(ckey_max, key_off, box_min, box_max, [1, 1, 1])
+        self.hh.set_boundaries([2] * 6)
+        call("rh_set_interpolation", int(interpol_var), int(interpol_type))
+        self.interpol_var = interpol_var
+        self.hh.set_grid_device(uold, uold, amesh.grid_records())
+        self.hh.insert_hash_all(1, amesh.ngrid)
+        for l in amesh.levels:
+            self.hh.build_nbor(amesh.head[l], amesh.noct[l], l)
+        self.t = 0.0
+        self.nstep = 0
+
+    def courant(self, l):
+        return self.hh.cmpdt(self.am.head[l], self.am.noct[l], self.am.dx(l), self.p)
+
+    def step(self, dt=None):
+        am = self.am
+        dts = []
+        for l in am.levels:
+            c = self.courant(l)
+            dts.append(min(am.dx(l) / self.p.smallc, c["dt"]))
+            self.hh.set_unew(am.head[l], am.noct[l])
+        if dt is None:
+            dt = min(dts)
+        for l in reversed(am.levels):
+            self.hh.godunov(am.head[l], am.noct[l], l, am.levelmin, am.levelmax, dt, am.dx(l), self.p,
+                            precision=self.precision)
+            self.hh.set_uold(am.head[l], am.noct[l])
+            if l < am.levelmax:
+                call("rh_upload", am.head[l + 1], am.noct[l + 1], int(self.interpol_var == 1), self.p.gamma,
+                     self.p.smallr, self.p.smallc ** 2)
+        self.t += dt
+        self.nstep += 1
+        return dt
+
+    def state(self):
+        return self.hh.transfer_grid_host(self.am.ngrid)
+
+
+def amesh_level_params(amesh):
+    from .mesh import level_params
+    return level_params(amesh.levelmax, 3)

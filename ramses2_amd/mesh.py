@@ -130,3 +130,68 @@ def prim_to_cons(rho, vel, p, gamma):
     vx, vy, vz = vel
     ekin = 0.5 * rho * (vx ** 2 + vy ** 2 + vz ** 2)
     return np.stack([rho, rho * vx, rho * vy, rho * vz, p / (gamma - 1.0) + ekin])
+
+
+class AmrMesh:
+    """Several levels of octs in the reference layout: levels stored
+    contiguously from coarse to fine (head(l), noct(l)), Morton order inside a
+    level, refined(ind) set where the child oct exists (an oct of level l+1
+    with key K is the child of the level-l cell with coordinates K)."""
+
+    def __init__(self, ndim, level_keys, boxlen=1.0):
+        assert ndim == 3, "AmrMesh builds 3D meshes"
+        self.ndim = ndim
+        self.boxlen = boxlen
+        self.ttd = 8
+        self.levels = sorted(level_keys)
+        self.levelmin, self.levelmax = self.levels[0], self.levels[-1]
+        ck, lv = [], []
+        self.head, self.noct = {}, {}
+        pos = 0
+        for l in self.levels:
+            k = np.asarray(level_keys[l], dtype=np.int64).reshape(-1, 3)
+            order = np.argsort(morton3(k[:, 0], k[:, 1], k[:, 2]), kind="stable")
+            k = k[order]
+            self.head[l] = pos + 1
+            self.noct[l] = len(k)
+            pos += len(k)
+            ck.append(k)
+            lv.append(np.full(len(k), l))
+        self.ckey = np.concatenate(ck).astype(np.int32)
+        self.lev = np.concatenate(lv).astype(np.int32)
+        self.ngrid = pos
+        self.refined = np.zeros((pos, 8), dtype=np.uint8)
+        for l in self.levels[:-1]:
+            fine = set(map(tuple, self.ckey[self.head[l + 1] - 1:self.head[l + 1] - 1 + self.noct[l + 1]]))
+            for i in range(self.head[l] - 1, self.head[l] - 1 + self.noct[l]):
+                c = self.ckey[i]
+                for ind in range(8):
+                    cc = (2 * c[0] + (ind & 1), 2 * c[1] + ((ind >> 1) & 1), 2 * c[2] + ((ind >> 2) & 1))
+                    if cc in fine:
+                        self.refined[i, ind] = 1
+
+    def dx(self, l):
+        return self.boxlen / 2 ** l
+
+    def grid_records(self):
+        g = np.zeros(self.ngrid, dtype=OCT3)
+        g["ckey"] = self.ckey
+        g["lev"] = self.lev
+        g["refined"] = self.refined
+        return g
+
+    def heads(self):
+        """head/noct arrays indexed by level-1 for levels 1..levelmax."""
+        head = np.zeros(self.levelmax, dtype=np.int32)
+        noct = np.zeros(self.levelmax, dtype=np.int32)
+        for l in self.levels:
+            head[l - 1] = self.head[l]
+            noct[l - 1] = self.noct[l]
+        return head, noct
+
+    def cell_centres(self, l):
+        """(noct(l), 8, 3) cell-centre coordinates of level l."""
+        i0 = self.head[l] - 1
+        k = self.ckey[i0:i0 + self.noct[l]].astype(np.float64)
+        off = np.array([[(ind & 1), (ind >> 1) & 1, (ind >> 2) & 1] for ind in range(8)], dtype=np.float64)
+        return (2 * k[:, None, :] + off[None, :, :] + 0.5) * self.dx(l)
