@@ -42,6 +42,15 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay steps as a hipGraph (no per-kernel events)")
+    ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs of the ensemble serving stream")
+    ap.add_argument("--serve-load", type=float, default=0.8,
+                    help="offered load as a fraction of the measured FP64 batched capacity")
+    ap.add_argument("--serve-steps", type=int, default=100)
+    ap.add_argument("--serve-slots", type=int, default=64)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (calibrates HBM byte counters)")
     return ap.parse_args()
 
 
@@ -117,6 +126,65 @@ def run_gpu(args, p, cells, mesh, precision):
     return lev, el, kms, dt_last, t_tot, sums
 
 
+def run_serving(args):
+    """Ensemble serving: capacity of the batched FP64 step measured on a closed
+    batch of the stream's first jobs, then the Poisson stream of all jobs at
+    serve_load of that capacity, served in FP64 and in mixed precision."""
+    from ramses2_amd import _lib
+    from ramses2_amd.serving import EnsembleServer, HipEngine, latency_summary, make_stream, mean_job_cells
+
+    base = make_stream(args.serve_jobs, rate=1.0, seed=args.seed, nsteps=args.serve_steps)
+    budget = int(args.serve_budget_gb * (1 << 30))
+    # capacity: serve_slots jobs of the stream resident, 20 batched steps timed
+    # (admission timed apart: it is host work between ticks)
+    cal = base[:args.serve_slots]
+    eng = HipEngine(precision=0)
+    t0 = time.perf_counter()
+    hs = [eng.open(s) for s in cal]
+    eng.sync()
+    t_open = (time.perf_counter() - t0) / len(cal)
+    eng.step(hs, 2)
+    eng.sync()
+    t0 = time.perf_counter()
+    ncal = 20
+    for _ in range(ncal):
+        eng.step(hs)
+    eng.sync()
+    el = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for h in hs:
+        eng.close(h)
+    t_close = (time.perf_counter() - t0) / len(cal)
+    capacity = sum(s.cells for s in cal) * ncal / el
+    mean_cells = mean_job_cells(base)
+    rate = args.serve_load * capacity / (mean_cells * args.serve_steps)
+    for s in base:
+        s.arrival = s.arrival / rate
+    print("bench: serving capacity %.3g cell-updates/s (%d jobs resident, batched steps); admission %.2f ms, "
+          "release %.2f ms per job; rate %.2f jobs/s for load %.2f"
+          % (capacity, len(cal), t_open * 1e3, t_close * 1e3, rate, args.serve_load), file=sys.stderr, flush=True)
+    out = {"jobs": len(base), "arrival_rate_jobs_per_s": rate, "offered_load": args.serve_load,
+           "capacity_cell_updates_per_s": capacity, "capacity_note": "%d jobs of the stream resident, %d batched "
+           "FP64 steps timed" % (len(cal), ncal), "admission_ms_per_job": t_open * 1e3,
+           "release_ms_per_job": t_close * 1e3, "mean_job_cells": mean_cells, "steps_per_job": args.serve_steps,
+           "max_slots": args.serve_slots, "hbm_budget_bytes": budget,
+           "sizes": "N^3, N log-uniform in [32,128], even; 50% blast, 50% Riemann; gamma 1.4, MonCen, HLLC, C 0.5"}
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        eng = HipEngine(precision=prec)
+        srv = EnsembleServer(eng, max_slots=args.serve_slots, hbm_budget=budget)
+        res = srv.run(base)
+        summ = latency_summary(res, base)
+        tr = np.array(srv.trace)
+        summ.update({"ticks": srv.ticks, "mean_resident_jobs": float(tr[:, 1].mean()) if len(tr) else 0.0,
+                     "max_bytes_in_use": int(tr[:, 2].max()) if len(tr) else 0})
+        out[name] = summ
+        print("bench: serving %s p50 %.3f s p99 %.3f s, %.3g cell-updates/s over %.1f s"
+              % (name, summ["p50_s"], summ["p99_s"], summ["throughput_cell_updates_per_s"], summ["span_s"]),
+              file=sys.stderr, flush=True)
+    _lib.call("rh_ctx_select", 0)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -134,6 +202,16 @@ def main():
     ncells = mesh.noct * 8
     print("bench: turbulence %d^3 (%d cells), warmup %d, steps %d" % (n, ncells, args.warmup, args.steps),
           file=sys.stderr, flush=True)
+
+    if args.copy_probe_mb > 0:
+        import ctypes
+        from ramses2_amd import _lib
+        _lib.call("rh_init")
+        ms = ctypes.c_float()
+        nbytes = args.copy_probe_mb << 20
+        _lib.call("rh_copy_probe", ctypes.c_int64(nbytes), ctypes.byref(ms))
+        print("bench: copy probe %d B read + %d B written in %.3f ms (%.0f GB/s)"
+              % (nbytes, nbytes, ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9), file=sys.stderr, flush=True)
 
     lev64, el64, kms64, dt64, t64, sums64 = run_gpu(args, p, cells, mesh, precision=0)
     u64 = lev64.state()
@@ -167,6 +245,10 @@ def main():
                 "kernel": "k_godunov3d<7,7,8,double> (FP64)", "kernel_ms": kms64,
                 "bytes_per_launch": BYTES_PER_CELL * ncells}
 
+    serving = None
+    if not args.no_serve:
+        serving = run_serving(args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, p)
@@ -192,6 +274,7 @@ def main():
         "mixed": mixed,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out))
 

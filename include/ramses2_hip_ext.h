@@ -19,6 +19,12 @@ extern "C" {
  * conserved variables, FP64 dt reduction and conservation sums. */
 int rh_set_precision(int precision);
 
+/* Courant pass of the next step inside the Godunov kernel (rh_level_run, 3D):
+ * -1 automatic (on for precision 1 only, where it measures faster), 0 a
+ * separate Courant kernel before each step, 1 the epilogue of the Godunov
+ * kernel reduces the Courant condition of the state it has just written. */
+int rh_set_fused_courant(int mode);
+
 /* Oct-brick shape (octs per brick in x, y, z) used by the next rh_build_nbor
  * for 3D levels; the autotuner (ramses2_amd/autotune.py) picks it. */
 int rh_set_brick_shape(int bx, int by, int bz);
@@ -51,6 +57,70 @@ int rh_copy_probe(int64_t nbytes, float *ms);
 int rh_time_godunov(int head_idx, int num_octs, int ilevel, double dx, double gamma, double smallr,
                     double smallc2, double courant_factor, const double g[3], int slope, int riemann,
                     double dt, int nlaunch, float *ms_per_launch);
+
+/* ---- Batched AMR stepping -----------------------------------------------------
+ * Prepare the levels levelmin..levelmax of the current context (octs of level l
+ * at head[l-1] .. head[l-1]+noct[l-1]-1, levels contiguous coarse to fine, each
+ * built with rh_build_nbor) for batched steps with one shared dt: stores one
+ * Godunov descriptor per level and the work list of all their bricks, and
+ * computes the first dt (courant_fine over the leaf cells of all levels,
+ * min with dx_finest/smallc).  dx of level l is boxlen/2^l. */
+int rh_amr_prepare(int levelmin, int levelmax, const int32_t *head, const int32_t *noct, double boxlen,
+                   double gamma, double smallr, double smallc2, double courant_factor, const double g[3],
+                   int slope, int riemann);
+
+/* nsteps steps of every prepared level: ghost octs refreshed from the parent
+ * level, one work-list Godunov launch over the bricks of all levels (refluxes
+ * gathered per ghost oct, then added to the parent cells), upload_fine finest
+ * first, courant_fine of the new state over all levels (dt of the next step).
+ * use_graph != 0 replays pairs of steps as a captured hipGraph.  Returns the dt
+ * of the next step, the time advanced since rh_amr_prepare, the conservation
+ * sums over all leaf cells {mass, ekin, eint, emag}, and the mean step time
+ * (HIP events on the library stream). */
+int rh_amr_run(int nsteps, int precision, int use_graph, double *dt_last, double *t_total, double sums[4],
+               float *ms_per_step);
+
+/* ---- Contexts and ensemble serving ----------------------------------------
+ * A context holds one mesh (arrays, hash, neighbour and brick tables, dt and
+ * time on the device).  Context 0 always exists; every entry point of
+ * ramses2_hip.h acts on the current context.  All contexts share the library
+ * stream, so work of different jobs can share launches. */
+int rh_ctx_create(int32_t *id);          /* new, empty context (not selected)      */
+int rh_ctx_select(int id);               /* make it current                         */
+int rh_ctx_current(int32_t *id);
+int rh_ctx_destroy(int id);              /* free its device memory; id 0 is reset   */
+int rh_ctx_footprint(int id, int64_t *bytes);   /* device bytes the context holds  */
+
+/* Device bytes a context needs for a uniform 3D level of n^3 octs (n octs per
+ * dimension) with the current brick shape: the admission estimate of the
+ * serving loop (rh_ctx_footprint reports the bytes actually held). */
+int rh_estimate_uniform_bytes(int n_octs_per_dim, int ncachemax, int hash_size, int64_t *bytes);
+
+/* Initial state of octs head_idx.. of the current context, written on the
+ * device into uold and unew (ncell cells per dimension, gamma):
+ *   kind 1, blast:   params = {rho0, p0, p_blast, lo, -}: (rho0, 0, p0), and p_blast in
+ *                    the 2^3 cells with every integer coordinate in [lo, lo+2);
+ *   kind 2, Riemann: params = {rho_l, p_l, rho_r, p_r, axis}: left state where
+ *                    (i_axis + 0.5)/ncell < 0.5, velocity 0. */
+int rh_init_ic(int head_idx, int num_octs, int ncell, int kind, const double params[5], double gamma);
+
+/* Make the current context's level a job for rh_batch_step: stores its Godunov
+ * parameters and computes the dt of its first step with a Courant pass; the
+ * job time starts at 0.  The level must be built (rh_build_nbor). */
+int rh_job_prepare(int head_idx, int num_octs, int ilevel, double dx, double gamma, double smallr,
+                   double smallc2, double courant_factor, const double g[3], int slope, int riemann);
+
+/* nsteps time steps of each of the njobs prepared contexts ids[] with shared
+ * launches: per step one work-list Godunov kernel over the bricks of
+ * all the jobs (each job's Courant condition of the new state reduced in the
+ * kernel's epilogue) and one per-job final reduction (dt of the job's next
+ * step, its time).  Jobs advance independently: a job's result does not depend
+ * on the other jobs of the batch.  Asynchronous. */
+int rh_batch_step(const int32_t *ids, int njobs, int precision, int nsteps);
+
+/* Synchronous read of a job's {t, dt of the next step, mass, ekin, eint, emag,
+ * dt} (the last four from the last Courant reduction). */
+int rh_job_status(int id, double out[7]);
 
 #ifdef __cplusplus
 }

@@ -45,7 +45,9 @@ class Mesh(ctypes.Structure):
                 ("head", _ip), ("noct", _ip), ("boxlen", ctypes.c_double),
                 ("periodic", ctypes.c_int * 3), ("bound_type", (ctypes.c_int * 2) * 3),
                 ("interpol_var", ctypes.c_int), ("interpol_type", ctypes.c_int),
-                ("constant_gravity", ctypes.c_double * 3)]
+                ("constant_gravity", ctypes.c_double * 3),
+                ("box_lev", ctypes.c_int), ("box_ckey_min", ctypes.c_int * 3),
+                ("box_ckey_max", ctypes.c_int * 3)]
 
 
 def _ptr(a, t):
@@ -168,7 +170,9 @@ class OracleMesh:
 
     def __init__(self, ndim, ckey, lev, refined, uold, head, noct, levelmin, boxlen=1.0,
                  periodic=(1, 1, 1), bound_type=((2, 2), (2, 2), (2, 2)),
-                 interpol_var=0, interpol_type=1, gravity=(0.0, 0.0, 0.0), unew=None):
+                 interpol_var=0, interpol_type=1, gravity=(0.0, 0.0, 0.0), unew=None, box=None):
+        """box = (lev, ckey_min[3], ckey_max[3]) for a computational box smaller
+        than the full 2^(lev-1)-oct level (box_ckey_min/max of the reference)."""
         ngrid = len(lev)
         self.ndim = ndim
         self.ckey = np.zeros((ngrid, 3), dtype=np.int32)
@@ -198,6 +202,11 @@ class OracleMesh:
             m.bound_type[d][1] = int(bound_type[d][1])
             m.constant_gravity[d] = gravity[d]
         m.interpol_var, m.interpol_type = interpol_var, interpol_type
+        if box is not None:
+            m.box_lev = int(box[0])
+            for d in range(3):
+                m.box_ckey_min[d] = int(box[1][d])
+                m.box_ckey_max[d] = int(box[2][d])
         self.m = m
 
     def godunov_fine(self, ilevel, dt, p):

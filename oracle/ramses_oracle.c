@@ -780,6 +780,20 @@ typedef struct {
 
 static int32_t box_max(int lev) { return lev >= 1 ? (1 << (lev - 1)) : 1; }
 
+/* box_ckey_min/max(d, lev) (amr/init_amr.f90:250-280) */
+static int64_t box_lo(const ora_mesh_t *m, int lev, int d)
+{
+    if (m->box_lev <= 0) return 0;
+    return lev >= m->box_lev ? (int64_t)m->box_ckey_min[d] << (lev - m->box_lev)
+                             : (int64_t)m->box_ckey_min[d] >> (m->box_lev - lev);
+}
+static int64_t box_hi(const ora_mesh_t *m, int lev, int d)
+{
+    if (m->box_lev <= 0) return box_max(lev);
+    return lev >= m->box_lev ? (int64_t)m->box_ckey_max[d] << (lev - m->box_lev)
+                             : (int64_t)m->box_ckey_max[d] >> (m->box_lev - lev);
+}
+
 /* get_grid for a (possibly out-of-domain) key: returns an oct index (>0),
  * -1 when a boundary ghost was built into *g, or 0 when missing.             */
 static int get_grid(const lookup_t *L, int lev, int64_t *key, ghost_t *g)
@@ -790,8 +804,8 @@ static int get_grid(const lookup_t *L, int lev, int64_t *key, ghost_t *g)
     if (c > 0) return c;
     int in_domain = 1, bdim = -1, side = 0;
     for (int d = 0; d < ndim; d++) {
-        if (key[d] < 0) { in_domain = 0; if (bdim < 0) { bdim = d; side = 0; } }
-        else if (key[d] >= box_max(lev)) { in_domain = 0; if (bdim < 0) { bdim = d; side = 1; } }
+        if (key[d] < box_lo(m, lev, d)) { in_domain = 0; if (bdim < 0) { bdim = d; side = 0; } }
+        else if (key[d] >= box_hi(m, lev, d)) { in_domain = 0; if (bdim < 0) { bdim = d; side = 1; } }
     }
     if (in_domain) return 0;
     /* boundary ghost: reference oct one step back into the domain */
@@ -833,8 +847,8 @@ static void wrap_key(const ora_mesh_t *m, int lev, int64_t *key)
 {
     for (int d = 0; d < m->ndim; d++)
         if (m->periodic[d]) {
-            if (key[d] < 0) key[d] = box_max(lev) - 1;
-            if (key[d] >= box_max(lev)) key[d] = 0;
+            if (key[d] < box_lo(m, lev, d)) key[d] = box_hi(m, lev, d) - 1;
+            if (key[d] >= box_hi(m, lev, d)) key[d] = box_lo(m, lev, d);
         }
 }
 
@@ -1195,6 +1209,7 @@ int ora_godunov_uniform_omp(const ora_mesh_t *m, int ilevel, double dt, const or
     level_range(m, ilevel, &head, &noct);
     int ndim = m->ndim, nvar = m->nvar, ttd = 1 << ndim;
     if (ndim != 3) return 3;
+    if (m->box_lev > 0) return 4; /* full periodic boxes only */
     int nx = box_max(ilevel);
     double dx = m->boxlen / (double)(1LL << ilevel);
     if (nthreads < 1) nthreads = 1;

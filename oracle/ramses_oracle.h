@@ -54,6 +54,11 @@ typedef struct {
     int bound_type[3][2];     /* [dim][0=low side,1=high side]             */
     int interpol_var, interpol_type;
     double constant_gravity[3];
+    /* Computational box as box_ckey_min/max (amr/init_amr.f90:250-280): at
+     * level l the box is [box_ckey_min*2^(l-box_lev), box_ckey_max*2^(l-box_lev))
+     * per dimension.  box_lev = 0 selects the full box [0, 2^(l-1)).          */
+    int box_lev;
+    int box_ckey_min[3], box_ckey_max[3];
 } ora_mesh_t;
 
 /* Riemann solvers, hydro/godunov_utils.f90:338-857.  q[5] = (rho,u_n,u_t1,u_t2,p);

@@ -14,13 +14,51 @@
 
 namespace rh {
 
-static State g_state;
-State &st() { return g_state; }
-void set_error(const std::string &msg) { g_state.last_error = msg; }
+// Contexts: each holds one mesh with its device arrays and tables.  Context 0
+// always exists; the drop-in entry points act on the current context.
+static State g_default;
+static State *g_cur = &g_default;
+static std::vector<State *> g_ctx{&g_default};
+static hipStream_t g_stream = nullptr;  // library stream, shared by all contexts
+State &st() { return *g_cur; }
+void set_error(const std::string &msg) { g_cur->last_error = msg; }
+
+hipError_t dmalloc(void **p, size_t n)
+{
+    hipError_t e = hipMalloc(p, n);
+    if (e == hipSuccess) {
+        g_cur->allocs[*p] = n;
+        g_cur->dev_bytes += (int64_t)n;
+    } else {
+        *p = nullptr;
+    }
+    return e;
+}
+
+void dfree(void *p)
+{
+    if (!p) return;
+    auto it = g_cur->allocs.find(p);
+    if (it != g_cur->allocs.end()) {
+        g_cur->dev_bytes -= (int64_t)it->second;
+        g_cur->allocs.erase(it);
+    }
+    (void)hipFree(p);
+}
+
+int ctx_count() { return (int)g_ctx.size(); }
+State *ctx_get(int id) { return id >= 0 && id < (int)g_ctx.size() ? g_ctx[id] : nullptr; }
+int ctx_add(State *s)
+{
+    g_ctx.push_back(s);
+    return (int)g_ctx.size() - 1;
+}
+void ctx_set_slot(int id, State *s) { g_ctx[id] = s; }
+void ctx_select(State *s) { g_cur = s; }
 
 static int fail(int code, const std::string &msg)
 {
-    g_state.last_error = msg;
+    st().last_error = msg;
     return code;
 }
 
@@ -283,34 +321,34 @@ __global__ void k_build_father(const int32_t *__restrict__ ckey, int head, int n
 // ---------------------------------------------------------------------------
 static int ensure_init()
 {
-    if (!g_state.inited) return fail(RH_ERR_STATE, "rh_init has not been called");
+    if (!st().inited) return fail(RH_ERR_STATE, "rh_init has not been called");
     return 0;
 }
 
 static int ensure_mesh()
 {
     if (int r = ensure_init()) return r;
-    if (!g_state.uold) return fail(RH_ERR_STATE, "rh_alloc_amr has not been called");
+    if (!st().uold) return fail(RH_ERR_STATE, "rh_alloc_amr has not been called");
     return 0;
 }
 
 static void free_level_tables()
 {
-    for (auto &kv : g_state.bricks) {
-        (void)hipFree(kv.second.win);
-        (void)hipFree(kv.second.org);
+    for (auto &kv : st().bricks) {
+        dfree(kv.second.win);
+        dfree(kv.second.org);
     }
-    g_state.bricks.clear();
-    for (auto &kv : g_state.nbors) {
-        (void)hipFree(kv.second.nbor);
-        (void)hipFree(kv.second.father);
+    st().bricks.clear();
+    for (auto &kv : st().nbors) {
+        dfree(kv.second.nbor);
+        dfree(kv.second.father);
     }
-    g_state.nbors.clear();
+    st().nbors.clear();
 }
 
 static int level_of_range(int head_idx, int num_octs, int &ilevel)
 {
-    State &s = g_state;
+    State &s = st();
     if (head_idx < 1 || num_octs < 0 || (int64_t)head_idx - 1 + num_octs > s.ngridtot)
         return fail(RH_ERR_ARG, "oct range out of bounds");
     if (num_octs == 0) {
@@ -325,7 +363,7 @@ static int level_of_range(int head_idx, int num_octs, int &ilevel)
 static ResolveCtx make_resolve(int ilevel, const int32_t box_min[3], const int32_t box_max[3],
                                const int32_t periodic[3])
 {
-    State &s = g_state;
+    State &s = st();
     ResolveCtx c;
     c.keys = s.hkeys;
     c.vals = s.hvals;
@@ -350,24 +388,25 @@ using namespace rh;
 // ---------------------------------------------------------------------------
 extern "C" {
 
-const char *rh_last_error(void) { return g_state.last_error.c_str(); }
+const char *rh_last_error(void) { return st().last_error.c_str(); }
 
 int rh_init(void)
 {
-    State &s = g_state;
+    State &s = st();
     if (s.inited) return 0;
     int n = 0;
     RH_HIP_CHECK(hipGetDeviceCount(&n));
     if (n < 1) return fail(RH_ERR_HIP, "no HIP device visible");
     RH_HIP_CHECK(hipGetDevice(&s.device));
-    RH_HIP_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
-    RH_HIP_CHECK(hipMalloc(&s.red_out, 8 * sizeof(double)));
+    if (!g_stream) RH_HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    s.stream = g_stream;
+    RH_HIP_CHECK(dmalloc((void **)&s.red_out, 8 * sizeof(double)));
     RH_HIP_CHECK(hipHostMalloc(&s.h_red, 8 * sizeof(double), hipHostMallocDefault));
     s.red_cap = 4096;
-    RH_HIP_CHECK(hipMalloc(&s.red_part, (size_t)s.red_cap * 4 * sizeof(double)));
-    RH_HIP_CHECK(hipMalloc(&s.dt_dev, 2 * sizeof(double)));
-    RH_HIP_CHECK(hipMalloc(&s.d_counter, 4 * sizeof(int)));
-    RH_HIP_CHECK(hipMalloc(&s.t_dev, 2 * sizeof(double)));
+    RH_HIP_CHECK(dmalloc((void **)&s.red_part, (size_t)s.red_cap * 4 * sizeof(double)));
+    RH_HIP_CHECK(dmalloc((void **)&s.dt_dev, 2 * sizeof(double)));
+    RH_HIP_CHECK(dmalloc((void **)&s.d_counter, 4 * sizeof(int)));
+    RH_HIP_CHECK(dmalloc((void **)&s.t_dev, 2 * sizeof(double)));
     RH_HIP_CHECK(hipEventCreate(&s.ev0));
     RH_HIP_CHECK(hipEventCreate(&s.ev1));
     s.inited = true;
@@ -377,30 +416,31 @@ int rh_init(void)
 
 int rh_finalize(void)
 {
-    State &s = g_state;
+    State &s = st();
     if (!s.inited) return 0;
     (void)hipStreamSynchronize(s.stream);
     free_level_tables();
-    (void)hipFree(s.uold);
-    (void)hipFree(s.unew);
-    (void)hipFree(s.ckey);
-    (void)hipFree(s.lev);
-    (void)hipFree(s.refined);
-    (void)hipFree(s.hkeys);
-    (void)hipFree(s.hvals);
-    (void)hipFree(s.ghost_parent);
-    (void)hipFree(s.ghost_src);
-    (void)hipFree(s.ghost_srckey);
-    (void)hipFree(s.ghost_key);
-    (void)hipFree(s.d_counter);
-    (void)hipFree(s.red_part);
-    (void)hipFree(s.red_out);
+    dfree(s.uold);
+    dfree(s.unew);
+    dfree(s.ckey);
+    dfree(s.lev);
+    dfree(s.refined);
+    dfree(s.hkeys);
+    dfree(s.hvals);
+    dfree(s.ghost_parent);
+    dfree(s.ghost_src);
+    dfree(s.ghost_srckey);
+    dfree(s.ghost_key);
+    dfree(s.d_counter);
+    dfree(s.red_part);
+    dfree(s.red_out);
     (void)hipHostFree(s.h_red);
-    (void)hipFree(s.dt_dev);
-    (void)hipFree(s.t_dev);
+    dfree(s.dt_dev);
+    dfree(s.t_dev);
     (void)hipEventDestroy(s.ev0);
     (void)hipEventDestroy(s.ev1);
-    (void)hipStreamDestroy(s.stream);
+    // anything else the context still holds (the shared stream stays)
+    while (!s.allocs.empty()) dfree(s.allocs.begin()->first);
     s = State();
     return 0;
 }
@@ -408,7 +448,7 @@ int rh_finalize(void)
 int rh_alloc_amr(int ngridmax, int ncachemax, int nvar, int twotondim, int hash_size)
 {
     if (int r = ensure_init()) return r;
-    State &s = g_state;
+    State &s = st();
     if (nvar != 5) return fail(RH_ERR_ARG, "nvar must be 5 (hydro, no passive scalars)");
     if (twotondim != 2 && twotondim != 8) return fail(RH_ERR_ARG, "twotondim must be 2 (1D) or 8 (3D)");
     if (ngridmax < 1 || ncachemax < 0 || (int64_t)ngridmax + ncachemax > (1 << 29))
@@ -416,13 +456,13 @@ int rh_alloc_amr(int ngridmax, int ncachemax, int nvar, int twotondim, int hash_
     if (hash_size < 1) return fail(RH_ERR_ARG, "hash_size must be positive");
     (void)hipStreamSynchronize(s.stream);
     free_level_tables();
-    (void)hipFree(s.uold);
-    (void)hipFree(s.unew);
-    (void)hipFree(s.ckey);
-    (void)hipFree(s.lev);
-    (void)hipFree(s.refined);
-    (void)hipFree(s.hkeys);
-    (void)hipFree(s.hvals);
+    dfree(s.uold);
+    dfree(s.unew);
+    dfree(s.ckey);
+    dfree(s.lev);
+    dfree(s.refined);
+    dfree(s.hkeys);
+    dfree(s.hvals);
     s.ndim = twotondim == 8 ? 3 : 1;
     s.ttd = twotondim;
     s.nvar = nvar;
@@ -433,12 +473,12 @@ int rh_alloc_amr(int ngridmax, int ncachemax, int nvar, int twotondim, int hash_
     while (cap < hash_size) cap <<= 1;
     s.hash_cap = cap;
     const size_t nd = (size_t)s.ngridtot * nvar * twotondim;
-    if (hipMalloc(&s.uold, nd * sizeof(double)) != hipSuccess || hipMalloc(&s.unew, nd * sizeof(double)) != hipSuccess ||
-        hipMalloc(&s.ckey, (size_t)s.ngridtot * 3 * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&s.lev, (size_t)s.ngridtot * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&s.refined, (size_t)s.ngridtot * 8) != hipSuccess ||
-        hipMalloc(&s.hkeys, (size_t)cap * sizeof(uint64_t)) != hipSuccess ||
-        hipMalloc(&s.hvals, (size_t)cap * sizeof(int32_t)) != hipSuccess)
+    if (dmalloc((void **)&s.uold, nd * sizeof(double)) != hipSuccess || dmalloc((void **)&s.unew, nd * sizeof(double)) != hipSuccess ||
+        dmalloc((void **)&s.ckey, (size_t)s.ngridtot * 3 * sizeof(int32_t)) != hipSuccess ||
+        dmalloc((void **)&s.lev, (size_t)s.ngridtot * sizeof(int32_t)) != hipSuccess ||
+        dmalloc((void **)&s.refined, (size_t)s.ngridtot * 8) != hipSuccess ||
+        dmalloc((void **)&s.hkeys, (size_t)cap * sizeof(uint64_t)) != hipSuccess ||
+        dmalloc((void **)&s.hvals, (size_t)cap * sizeof(int32_t)) != hipSuccess)
         return fail(RH_ERR_NOMEM, "device allocation failed in rh_alloc_amr");
     RH_HIP_CHECK(hipMemsetAsync(s.uold, 0, nd * sizeof(double), s.stream));
     RH_HIP_CHECK(hipMemsetAsync(s.unew, 0, nd * sizeof(double), s.stream));
@@ -449,14 +489,14 @@ int rh_alloc_amr(int ngridmax, int ncachemax, int nvar, int twotondim, int hash_
     RH_HIP_CHECK(hipMemsetAsync(s.hvals, 0, (size_t)cap * sizeof(int32_t), s.stream));
     {
         const size_t ng = (size_t)(ncachemax > 0 ? ncachemax : 1);
-        (void)hipFree(s.ghost_parent);
-        (void)hipFree(s.ghost_src);
-        (void)hipFree(s.ghost_srckey);
-        (void)hipFree(s.ghost_key);
-        if (hipMalloc(&s.ghost_parent, ng * 2 * sizeof(int32_t)) != hipSuccess ||
-            hipMalloc(&s.ghost_src, ng * 14 * sizeof(int32_t)) != hipSuccess ||
-            hipMalloc(&s.ghost_srckey, ng * 21 * sizeof(int32_t)) != hipSuccess ||
-            hipMalloc(&s.ghost_key, ng * 3 * sizeof(int32_t)) != hipSuccess)
+        dfree(s.ghost_parent);
+        dfree(s.ghost_src);
+        dfree(s.ghost_srckey);
+        dfree(s.ghost_key);
+        if (dmalloc((void **)&s.ghost_parent, ng * 2 * sizeof(int32_t)) != hipSuccess ||
+            dmalloc((void **)&s.ghost_src, ng * 14 * sizeof(int32_t)) != hipSuccess ||
+            dmalloc((void **)&s.ghost_srckey, ng * 21 * sizeof(int32_t)) != hipSuccess ||
+            dmalloc((void **)&s.ghost_key, ng * 3 * sizeof(int32_t)) != hipSuccess)
             return fail(RH_ERR_NOMEM, "device allocation failed for the ghost-oct tables");
         s.nghost_total = 0;
     }
@@ -470,7 +510,7 @@ int rh_upload_level_params(const int32_t *ckey_max, const int64_t *key_off, cons
                            const int32_t *box_ckey_max, const int32_t periodic[3], int nlevelmax)
 {
     if (int r = ensure_init()) return r;
-    State &s = g_state;
+    State &s = st();
     if (nlevelmax < 1 || nlevelmax > 63) return fail(RH_ERR_ARG, "nlevelmax must be in [1, 63]");
     if (!ckey_max || !key_off || !box_ckey_min || !box_ckey_max || !periodic)
         return fail(RH_ERR_ARG, "null level parameter array");
@@ -490,7 +530,7 @@ int rh_set_boundaries(const int32_t bound_type[6])
         for (int side = 0; side < 2; side++) {
             const int t = bound_type[2 * d + side];
             if (t != 1 && t != 2) return fail(RH_ERR_ARG, "bound_type must be 1 (reflexive) or 2 (zero gradient)");
-            g_state.btype[d][side] = t;
+            st().btype[d][side] = t;
         }
     return 0;
 }
@@ -499,7 +539,7 @@ int rh_set_grid_device(const double *uold, const double *unew, const void *bold,
                        int nvar, int twotondim)
 {
     if (int r = ensure_mesh()) return r;
-    State &s = g_state;
+    State &s = st();
     if (bold) return fail(RH_ERR_ARG, "bold must be NULL (no MHD)");
     if (ngridmax < 0 || ngridmax > s.ngridtot || nvar != s.nvar || twotondim != s.ttd)
         return fail(RH_ERR_ARG, "rh_set_grid_device: sizes differ from rh_alloc_amr");
@@ -536,7 +576,7 @@ int rh_set_grid_device(const double *uold, const double *unew, const void *bold,
 int rh_transfer_grid_host(double *uold, void *bold, int ngridmax, int nvar, int twotondim)
 {
     if (int r = ensure_mesh()) return r;
-    State &s = g_state;
+    State &s = st();
     (void)bold;
     if (!uold || ngridmax < 0 || ngridmax > s.ngridtot || nvar != s.nvar || twotondim != s.ttd)
         return fail(RH_ERR_ARG, "rh_transfer_grid_host: bad arguments");
@@ -548,7 +588,7 @@ int rh_transfer_grid_host(double *uold, void *bold, int ngridmax, int nvar, int 
 int rh_transfer_unew_host(double *unew, int ngridmax, int nvar, int twotondim)
 {
     if (int r = ensure_mesh()) return r;
-    State &s = g_state;
+    State &s = st();
     if (!unew || ngridmax < 0 || ngridmax > s.ngridtot || nvar != s.nvar || twotondim != s.ttd)
         return fail(RH_ERR_ARG, "rh_transfer_unew_host: bad arguments");
     RH_HIP_CHECK(hipMemcpyAsync(unew, s.unew, (size_t)nvar * twotondim * ngridmax * sizeof(double), hipMemcpyDeviceToHost, s.stream));
@@ -559,7 +599,7 @@ int rh_transfer_unew_host(double *unew, int ngridmax, int nvar, int twotondim)
 int rh_insert_hash_all(int head_idx, int num_octs, int hash_size)
 {
     if (int r = ensure_mesh()) return r;
-    State &s = g_state;
+    State &s = st();
     (void)hash_size;
     if (s.nlevelmax < 1) return fail(RH_ERR_STATE, "rh_upload_level_params has not been called");
     if (head_idx < 1 || num_octs < 0 || (int64_t)head_idx - 1 + num_octs > s.ngridtot)
@@ -589,7 +629,7 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
                   const int32_t box_min[3], const int32_t box_max[3], const int32_t periodic[3])
 {
     if (int r = ensure_mesh()) return r;
-    State &s = g_state;
+    State &s = st();
     (void)hash_size;
     (void)ckey_max_l;
     (void)key_off_l;
@@ -608,8 +648,8 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
         if (it != s.nbors.end()) {
             prev_first = it->second.ghost_first;
             prev_n = it->second.nghost;
-            (void)hipFree(it->second.nbor);
-            (void)hipFree(it->second.father);
+            dfree(it->second.nbor);
+            dfree(it->second.father);
             s.nbors.erase(it);
         }
         LevelNbor ln;
@@ -619,7 +659,7 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
         ln.ghost_first = prev_first;
         ln.nghost = prev_n;
         const int nslot = s.ndim == 3 ? 27 : 3;
-        RH_HIP_CHECK(hipMalloc(&ln.nbor, (size_t)num_octs * nslot * sizeof(int32_t)));
+        RH_HIP_CHECK(dmalloc((void **)&ln.nbor, (size_t)num_octs * nslot * sizeof(int32_t)));
         const long n = (long)num_octs * nslot;
         const unsigned nblk = (unsigned)((n + 255) / 256);
         hipLaunchKernelGGL(k_build_nbor, dim3(nblk), dim3(256), 0, s.stream, s.ckey, head_idx, num_octs, c, nslot,
@@ -632,7 +672,7 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
         if (ilevel > 1) {
             ResolveCtx cc = make_resolve(ilevel - 1, &s.box_min[3 * (ilevel - 2)], &s.box_max[3 * (ilevel - 2)],
                                          periodic);
-            RH_HIP_CHECK(hipMalloc(&ln.father, (size_t)num_octs * 2 * sizeof(int32_t)));
+            RH_HIP_CHECK(dmalloc((void **)&ln.father, (size_t)num_octs * 2 * sizeof(int32_t)));
             hipLaunchKernelGGL(k_build_father, dim3((num_octs + 255) / 256), dim3(256), 0, s.stream, s.ckey,
                                head_idx, num_octs, cc, s.ndim, ln.father);
             RH_HIP_CHECK(hipGetLastError());
@@ -685,8 +725,8 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
     if (s.ndim == 3) {
         auto it = s.bricks.find(ilevel);
         if (it != s.bricks.end()) {
-            (void)hipFree(it->second.win);
-            (void)hipFree(it->second.org);
+            dfree(it->second.win);
+            dfree(it->second.org);
             s.bricks.erase(it);
         }
         LevelBricks lb;
@@ -718,8 +758,8 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
             org[3 * b + 2] = c.boxmin[2] + (int32_t)(k / (nbx * nby)) * lb.bz;
         }
         const int nw = (lb.bx + 2) * (lb.by + 2) * (lb.bz + 2);
-        RH_HIP_CHECK(hipMalloc(&lb.org, org.size() * sizeof(int32_t)));
-        RH_HIP_CHECK(hipMalloc(&lb.win, (size_t)lb.nbricks * nw * sizeof(int32_t)));
+        RH_HIP_CHECK(dmalloc((void **)&lb.org, org.size() * sizeof(int32_t)));
+        RH_HIP_CHECK(dmalloc((void **)&lb.win, (size_t)lb.nbricks * nw * sizeof(int32_t)));
         RH_HIP_CHECK(hipMemcpyAsync(lb.org, org.data(), org.size() * sizeof(int32_t), hipMemcpyHostToDevice, s.stream));
         const long n = (long)lb.nbricks * nw;
         hipLaunchKernelGGL(k_build_bricks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s.stream, lb.org,
@@ -755,7 +795,7 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
 int rh_set_unew(int head_idx, int num_octs)
 {
     if (int r = ensure_mesh()) return r;
-    State &s = g_state;
+    State &s = st();
     if (head_idx < 1 || num_octs < 0 || (int64_t)head_idx - 1 + num_octs > s.ngridtot)
         return fail(RH_ERR_ARG, "rh_set_unew: oct range out of bounds");
     const size_t per = (size_t)s.nvar * s.ttd;
@@ -767,7 +807,7 @@ int rh_set_unew(int head_idx, int num_octs)
 int rh_set_uold(int head_idx, int num_octs)
 {
     if (int r = ensure_mesh()) return r;
-    State &s = g_state;
+    State &s = st();
     if (head_idx < 1 || num_octs < 0 || (int64_t)head_idx - 1 + num_octs > s.ngridtot)
         return fail(RH_ERR_ARG, "rh_set_uold: oct range out of bounds");
     const size_t per = (size_t)s.nvar * s.ttd;
@@ -780,7 +820,7 @@ int rh_cmpdt(int head_idx, int num_octs, double dx, double gamma, double smallr,
              const double g[3], double *mass, double *ekin, double *eint, double *emag, double *dt)
 {
     if (int r = ensure_mesh()) return r;
-    State &s = g_state;
+    State &s = st();
     if (head_idx < 1 || num_octs < 0 || (int64_t)head_idx - 1 + num_octs > s.ngridtot)
         return fail(RH_ERR_ARG, "rh_cmpdt: oct range out of bounds");
     if (!(dx > 0) || !(smallc2 > 0) || !(gamma > 1)) return fail(RH_ERR_ARG, "rh_cmpdt: bad physical parameters");
@@ -803,7 +843,7 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
                const double g[3])
 {
     if (int r = ensure_mesh()) return r;
-    State &s = g_state;
+    State &s = st();
     (void)ngridmax;
     (void)levelmax;
     if (num_subgrids == 0) return 0;
@@ -879,6 +919,7 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         a.ghost_parent = s.ghost_parent;
         a.courant = 0;
         a.cpart = nullptr;
+        a.racc = nullptr;
         RH_HIP_CHECK(launch_godunov3d(a, lb.bx, lb.by, lb.bz, s.precision, s.stream));
     } else {
         const LevelNbor &ln = lnb;
@@ -919,15 +960,15 @@ int rh_set_interpolation(int interpol_var, int interpol_type)
 {
     if (interpol_var < 0 || interpol_var > 1 || interpol_type < 0 || interpol_type > 3)
         return fail(RH_ERR_ARG, "interpol_var must be 0/1 and interpol_type 0..3");
-    g_state.interpol_var = interpol_var;
-    g_state.interpol_type = interpol_type;
+    st().interpol_var = interpol_var;
+    st().interpol_type = interpol_type;
     return 0;
 }
 
 int rh_upload(int head_idx, int num_octs, int internal_energy, double gamma, double smallr, double smallc2)
 {
     if (int r = ensure_mesh()) return r;
-    State &s = g_state;
+    State &s = st();
     (void)gamma;
     (void)smallc2;
     if (num_octs == 0) return 0;
@@ -944,7 +985,7 @@ int rh_upload(int head_idx, int num_octs, int internal_energy, double gamma, dou
 int rh_device_sync(void)
 {
     if (int r = ensure_init()) return r;
-    RH_HIP_CHECK(hipStreamSynchronize(g_state.stream));
+    RH_HIP_CHECK(hipStreamSynchronize(st().stream));
     return 0;
 }
 

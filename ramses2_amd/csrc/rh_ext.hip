@@ -33,6 +33,10 @@ bool level_is_whole_mesh(int head_idx, int num_octs)
     return true;
 }
 
+}  // namespace
+
+namespace rh {
+
 // Fill the Godunov arguments of a 3D level in fused mode (unew = uold + dF).
 int fill_args3(int head_idx, int num_octs, int ilevel, double dx, double gamma, double smallr, double smallc2,
                double courant_factor, const double g[3], int slope, int riemann, const double *dt_dev, double dt,
@@ -40,8 +44,10 @@ int fill_args3(int head_idx, int num_octs, int ilevel, double dx, double gamma, 
 {
     State &s = st();
     auto it = s.bricks.find(ilevel);
-    if (it == s.bricks.end() || it->second.head != head_idx || it->second.noct != num_octs)
-        return efail(RH_ERR_STATE, "rh_build_nbor was not called for this level");
+    if (it == s.bricks.end() || it->second.head != head_idx || it->second.noct != num_octs) {
+        set_error("rh_build_nbor was not called for this level");
+        return RH_ERR_STATE;
+    }
     const LevelBricks &lb = it->second;
     lbo = &lb;
     a.uold = s.uold;
@@ -78,6 +84,7 @@ int fill_args3(int head_idx, int num_octs, int ilevel, double dx, double gamma, 
     a.ghost_parent = s.ghost_parent;
     a.courant = 0;
     a.cpart = nullptr;
+    a.racc = nullptr;
     return 0;
 }
 
@@ -87,8 +94,10 @@ int fill_args1(int head_idx, int num_octs, int ilevel, double dx, double gamma, 
 {
     State &s = st();
     auto it = s.nbors.find(ilevel);
-    if (it == s.nbors.end() || it->second.head != head_idx || it->second.noct != num_octs)
-        return efail(RH_ERR_STATE, "rh_build_nbor was not called for this level");
+    if (it == s.nbors.end() || it->second.head != head_idx || it->second.noct != num_octs) {
+        set_error("rh_build_nbor was not called for this level");
+        return RH_ERR_STATE;
+    }
     const LevelNbor &ln = it->second;
     a.uold = s.uold;
     a.unew = s.unew;
@@ -125,23 +134,37 @@ int ensure_partials(int n)
 {
     State &s = st();
     if (n <= s.red_cap) return 0;
-    (void)hipFree(s.red_part);
+    dfree(s.red_part);
     s.red_part = nullptr;
-    RH_HIP_CHECK(hipMalloc(&s.red_part, (size_t)n * 4 * sizeof(double)));
+    RH_HIP_CHECK(dmalloc((void **)&s.red_part, (size_t)n * 4 * sizeof(double)));
     s.red_cap = n;
     return 0;
 }
 
-// One step.  3D with a Courant dt: the Godunov kernel also reduces the
-// Courant condition of the state it writes (dt of the next step), followed by
-// a one-block final reduction; 1D or fixed dt: courant then godunov.
+}  // namespace rh
+
+namespace {
+
+// Whether the Courant pass of the next step runs in the Godunov epilogue.
+// Measured at 256^3 turbulence: the epilogue costs the FP64 kernel more than
+// the separate pass (higher register pressure), and saves the mixed kernel a
+// little, hence the automatic choice.
+bool courant_fused()
+{
+    const State &s = st();
+    return s.fused_courant < 0 ? s.precision == 1 : s.fused_courant != 0;
+}
+
+// One step.  3D with a Courant dt and fusion on: the Godunov kernel also
+// reduces the Courant condition of the state it writes (dt of the next step),
+// followed by a one-block final reduction; otherwise courant then godunov.
 hipError_t enqueue_step(int head_idx, int num_octs, double dx, double gamma, double smallr, double smallc2,
                         double courant_factor, const double g[3], double dt_fixed, GodArgs *a3, God1Args *a1,
                         const LevelBricks *lb, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     State &s = st();
     const double gz[3] = {0, 0, 0};
-    if (a3 && dt_fixed <= 0.0) {
+    if (a3 && dt_fixed <= 0.0 && courant_fused()) {
         a3->uold = s.uold;
         a3->unew = s.unew;
         a3->ubase = s.uold;
@@ -190,6 +213,13 @@ int rh_set_precision(int precision)
 {
     if (precision != 0 && precision != 1) return efail(RH_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
     st().precision = precision;
+    return 0;
+}
+
+int rh_set_fused_courant(int mode)
+{
+    if (mode < -1 || mode > 1) return efail(RH_ERR_ARG, "fused Courant mode must be -1, 0 or 1");
+    st().fused_courant = mode;
     return 0;
 }
 

@@ -40,6 +40,20 @@ struct GodArgs {
     // (mass, ekin, eint, dt) at cpart[4*brick]
     int courant;
     double *cpart;
+    // when non-null, refluxes accumulate into racc[5*(ghost index)] (one slot
+    // per ghost oct, i.e. per parent cell) instead of into the parent cell of
+    // unew; a later fold adds them (several levels in one launch)
+    double *racc;
+};
+
+// Work list of bricks over several descriptors (levels / jobs): item i is
+// (descriptor index, brick index); range[0..8] splits the items into one
+// contiguous range per XCD.  Descriptors must stay unchanged while a launch
+// runs (the kernel reads them through the constant address space).
+struct WorkList {
+    const GodArgs *descs;
+    const int2 *items;
+    int range[9];
 };
 
 // 1D arguments: one thread per cell of the level, 3-slot neighbour table.
@@ -67,14 +81,37 @@ struct God1Args {
 // (0 = FP64, 1 = FP32 reconstruction/fluxes with FP64 accumulation).
 hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, hipStream_t s);
 hipError_t launch_godunov1d(const God1Args &a, int prec, hipStream_t s);
-// Persistent work-list launch: items[i] = (descriptor index, brick index).
-hipError_t launch_godunov3d_wl(const GodArgs *descs, const int2 *items, int nitems, int nblocks, int bx, int by,
-                               int bz, int prec, hipStream_t s);
+// Work-list launch: items[i] = (descriptor index, brick index), one workgroup per item.
+hipError_t launch_godunov3d_wl(const WorkList &w, int bx, int by, int bz, int slope, int riemann, int prec,
+                               hipStream_t s);
 
 // Courant reduction over cells of octs [head, head+noct): per-block partials,
 // then a fixed-order final pass -> out[0..4] = mass, ekin, eint, emag, dt.
 // Fixed-order reduction of fused-Courant partials: t_acc += the dt the step
 // used (*dt_inout on entry), then *dt_inout = min(dtmax, partial dts).
+// Per-job final Courant reduction of a batch (one block per job).
+struct CourantJob {
+    const double *part;   // per-brick partials of the job
+    int nparts;
+    double dtmax;         // dx / smallc
+    double *out;          // {mass, ekin, eint, emag, dt}
+    double *dt_inout;     // dt of the step just taken in, dt of the next step out
+    double *t_acc;        // time of the job, advanced by the dt taken
+};
+// courant_fine over the leaf cells of the contiguous oct range [head,
+// head+noct) spanning several levels: dx of each oct is boxlen*2^-lev.
+// dx_finest sets the initial dt bound dx/smallc.  Partials + final reduction
+// as launch_courant.
+hipError_t launch_courant_ml(const double *uold, const uint8_t *refined, const int32_t *lev, int head, int noct,
+                             double boxlen, double dx_finest, double gamma, double smallr, double smallc,
+                             double courant_factor, const double g[3], double *partials, int max_blocks, double *out,
+                             double *dt_out, hipStream_t s, double *t_acc);
+// racc[0 .. 5*n) = 0 (a kernel: safe inside captured graphs)
+hipError_t launch_zero(double *p, long n, hipStream_t s);
+// unew(parent cell of ghost g) += racc[5*g .. 5*g+4] for the ghosts g in [0, nghost)
+hipError_t launch_reflux_fold(double *unew, const double *racc, const int32_t *ghost_parent, int nghost, int ttd,
+                              hipStream_t s);
+hipError_t launch_courant_reduce_batch(const CourantJob *jobs, int njobs, hipStream_t s);
 hipError_t launch_courant_reduce(const double *partials, int nparts, double dtmax, double *out, double *dt_inout,
                                  double *t_acc, hipStream_t s);
 

@@ -101,10 +101,11 @@ __device__ __forceinline__ void courant_cell(const double u[5], const CourantK &
 // value v, whose oct key is okey.  Boundary ghosts (v<0) follow
 // init_bound_refine (amr/boundaries.f90:39-165): zero gradient copies the
 // adjacent interior cell, reflexive mirrors it and flips the normal momentum.
-__device__ __forceinline__ void gather_cell(const double *__restrict__ uold, const uint8_t *__restrict__ refined,
-                                            int v, const int okey[3], int ndim, const int boxmin[3],
-                                            const int boxmax[3], const int btype[3][2], int c[3], double u[5],
-                                            uint8_t &rf)
+// refl: bit 2*d+side set where that face of the domain is reflexive
+// (bound_type 1), else zero gradient (bound_type 2).
+__device__ __forceinline__ void gather_cell_m(const double *__restrict__ uold, const uint8_t *__restrict__ refined,
+                                              int v, const int okey[3], int ndim, const int boxmin[3],
+                                              const int boxmax[3], int refl, int c[3], double u[5], uint8_t &rf)
 {
     if (v > 0) {
         const int ind = c[0] + 2 * c[1] + 4 * c[2];
@@ -128,7 +129,7 @@ __device__ __forceinline__ void gather_cell(const double *__restrict__ uold, con
         if (okey[d] < boxmin[d]) { dir = d; side = 0; }
         else if (okey[d] >= boxmax[d]) { dir = d; side = 1; }
     }
-    const int type = btype[dir][side];
+    const int type = (refl >> (2 * dir + side)) & 1 ? 1 : 2;
     int s[3] = {c[0], c[1], c[2]};
     uint8_t rflag_src_valid;
     int rs[3] = {c[0], c[1], c[2]};
@@ -152,6 +153,24 @@ __device__ __forceinline__ void gather_cell(const double *__restrict__ uold, con
     for (int n = 0; n < 5; n++) u[n] = p[n * stride];
     if (type == 1) u[1 + dir] = -u[1 + dir];
     rf = rflag_src_valid ? refined[(size_t)(ref - 1) * 8 + rs[0] + 2 * rs[1] + 4 * rs[2]] : 0;
+}
+
+__device__ __forceinline__ int reflexive_mask(const int btype[3][2])
+{
+    int refl = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int sd = 0; sd < 2; sd++) refl |= (btype[d][sd] == 1) << (2 * d + sd);
+    return refl;
+}
+
+__device__ __forceinline__ void gather_cell(const double *__restrict__ uold, const uint8_t *__restrict__ refined,
+                                            int v, const int okey[3], int ndim, const int boxmin[3],
+                                            const int boxmax[3], const int btype[3][2], int c[3], double u[5],
+                                            uint8_t &rf)
+{
+    gather_cell_m(uold, refined, v, okey, ndim, boxmin, boxmax, reflexive_mask(btype), c, u, rf);
 }
 
 template <typename T>
@@ -214,9 +233,15 @@ __device__ __forceinline__ void slope27(const T (*sq)[5][NP], int sm, int s0, in
 // ghost's parent cell.  FP64 atomics: the order of the contributions to one
 // parent cell is not fixed (last-bit differences from the serial reference).
 __device__ __forceinline__ void reflux_add(double *unew, const int32_t *ghost_parent, int ngridmax, int vghost,
-                                           double sign, const double F[5], int ttd)
+                                           double sign, const double F[5], int ttd, double *racc = nullptr)
 {
     const int gi = vghost - ngridmax - 1;
+    if (racc) {
+        const double w = 1.0 / (double)ttd;
+#pragma unroll
+        for (int n = 0; n < 5; n++) atomicAdd(racc + 5 * (size_t)gi + n, sign * (F[n] * w));
+        return;
+    }
     const int igrid = ghost_parent[2 * gi], icell = ghost_parent[2 * gi + 1];
     if (igrid <= 0) return;  // parent missing: rh_build_nbor reported it
     const double w = 1.0 / (double)ttd;
@@ -243,8 +268,11 @@ __device__ __forceinline__ int ring_slot3(int t) { return (t + 3) % 3; }
 
 // One brick: the body shared by the per-level kernel and the work-list kernel.
 // RS = 3 compiles HLLC in; NS is the depth of the primitive ring.
-template <int BX, int BY, int BZ, typename T, int RS, int NS>
-__device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, BrickSmem<BX, BY, BZ, T, NS> &sm)
+// A is GodArgs (a kernel argument) or GodArgs in the constant address space
+// (a work-list descriptor): both are read-only for the kernel's lifetime, so
+// the compiler reloads their fields with scalar loads instead of holding them.
+template <int BX, int BY, int BZ, typename T, int RS, int NS, typename A>
+__device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY, BZ, T, NS> &sm)
 {
     constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
     constexpr int PX = 2 * BX + 4, PY = 2 * BY + 4, NP = PX * PY;
@@ -267,6 +295,12 @@ __device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, Bric
     const double inv_dx = a.inv_dx;
     const int boxmin[3] = {a.boxmin[0], a.boxmin[1], a.boxmin[2]};
     const int boxmax[3] = {a.boxmax[0], a.boxmax[1], a.boxmax[2]};
+    int refl = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int sd = 0; sd < 2; sd++) refl |= (a.btype[d][sd] == 1) << (2 * d + sd);
+    const double gvec[3] = {a.g[0], a.g[1], a.g[2]};
     __syncthreads();
 
     const int cx = tid % CX - 1, cy = tid / CX - 1;
@@ -293,7 +327,7 @@ __device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, Bric
             int c[3] = {px & 1, py & 1, t & 1};
             double u[5], q[5];
             uint8_t rf;
-            gather_cell(a.uold, a.refined, v, okey, 3, boxmin, boxmax, a.btype, c, u, rf);
+            gather_cell_m(a.uold, a.refined, v, okey, 3, boxmin, boxmax, refl, c, u, rf);
             prim_from_cons(u, q, a.smallr, smalle, a.gamma, gdt);
 #pragma unroll
             for (int n = 0; n < 5; n++) sq[st][n][i] = T(q[n]);
@@ -401,20 +435,20 @@ __device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, Bric
                 const int vl = wv(cx - 1, cy, tt), vh = wv(cx, cy, tt);
 #pragma unroll
                 for (int n = 0; n < 5; n++) F[n] = (double(fx[n]) * dt) * inv_dx;
-                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8);
-                if (vh > a.ngridmax && owned(cx - 1, cy, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8);
+                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc);
+                if (vh > a.ngridmax && owned(cx - 1, cy, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc);
             }
             if (wfy && !(myref || sref[s0][pc - PX])) {
                 const int vl = wv(cx, cy - 1, tt), vh = wv(cx, cy, tt);
 #pragma unroll
                 for (int n = 0; n < 5; n++) F[n] = (double(fy[n]) * dt) * inv_dx;
-                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8);
-                if (vh > a.ngridmax && owned(cx, cy - 1, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8);
+                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc);
+                if (vh > a.ngridmax && owned(cx, cy - 1, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc);
             }
             if (interior && tt >= 0 && tt <= 2 * BZ && !(myref || ref_prev)) {
                 const int vl = wv(cx, cy, tt - 1), vh = wv(cx, cy, tt);
-                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, fz, 8);
-                if (vh > a.ngridmax && owned(cx, cy, tt - 1, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, fz, 8);
+                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, fz, 8, a.racc);
+                if (vh > a.ngridmax && owned(cx, cy, tt - 1, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, fz, 8, a.racc);
             }
         }
         __syncthreads();
@@ -469,7 +503,7 @@ __device__ __forceinline__ void brick_body(const GodArgs &a, const int bid, Bric
         // courant_fine of the state just written (the dt of the next step):
         // the brick's own leaf cells, re-read from unew while still in cache,
         // then the brick's partials in a fixed reduction order (FP64)
-        const CourantK ck = make_courant_k(a.dx, 3, a.gamma, a.smallr, a.smallc2, a.courant_factor, a.g);
+        const CourantK ck = make_courant_k(a.dx, 3, a.gamma, a.smallr, a.smallc2, a.courant_factor, gvec);
         double c_mass = 0.0, c_ekin = 0.0, c_eint = 0.0, c_dt = a.dx / ck.smallc;
         __syncthreads();
         for (int i = tid; i < BX * BY * BZ * 8; i += NT) {
@@ -535,20 +569,30 @@ __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), RH_BRICK_MIN_WAVE
     brick_body<BX, BY, BZ, T, RS, NS>(a, bid, sm);
 }
 
-// Persistent work-list kernel: work item i = (descriptor, brick).  Descriptors
-// carry the arrays and parameters of one level of one job, so bricks of
-// several levels and of several jobs share one launch.
+// Work-list kernel: work item i = (descriptor, brick), where a descriptor
+// carries the arrays and parameters of one level of one mesh, so bricks of
+// several levels and of several jobs share one launch.  The items are split
+// into 8 contiguous ranges, one per XCD: workgroups are placed on the XCDs
+// round-robin by workgroup id, so workgroup b runs item range[b % 8] + b / 8
+// on XCD b % 8, and neighbouring bricks (which share halo octs) meet in that
+// XCD's L2.  One workgroup per item: the hardware dispatcher is the queue.  (A
+// persistent loop over an atomic counter holds its loop state and the
+// reloaded descriptor in registers and measured 160 instead of 125 VGPRs for
+// the FP32 HLLC kernel, one wave per SIMD less.)
+typedef const __attribute__((address_space(4))) GodArgs ConstGodArgs;
+
 template <int BX, int BY, int BZ, typename T, int RS, int NS>
-__global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2))
-    k_godunov3d_wl(const GodArgs *__restrict__ descs, const int2 *__restrict__ items, int nitems)
+__global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), RH_BRICK_MIN_WAVES) k_godunov3d_wl(const WorkList w)
 {
     __shared__ BrickSmem<BX, BY, BZ, T, NS> sm;
-    for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
-        const int2 it = items[i];
-        brick_body<BX, BY, BZ, T, RS, NS>(descs[it.x], it.y, sm);
-        __syncthreads();
-    }
+    const int x = blockIdx.x & 7, l = blockIdx.x >> 3;
+    const int i = w.range[x] + l;
+    if (i >= w.range[x + 1]) return;
+    const int2 it = w.items[i];
+    const int di = __builtin_amdgcn_readfirstlane(it.x), bi = __builtin_amdgcn_readfirstlane(it.y);
+    brick_body<BX, BY, BZ, T, RS, NS>(((ConstGodArgs *)w.descs)[di], bi, sm);
 }
+
 
 // ---------------------------------------------------------------------------
 // 1D: one thread per cell; stencil of 5 cells from the 3-slot neighbour table.
@@ -664,11 +708,83 @@ __global__ void __launch_bounds__(kCourantThreads) k_courant(const double *__res
     }
 }
 
-// Fixed-order reduction of the per-block partials: out = {mass, ekin, eint,
-// emag, dt}; dt_out (optional) receives dt for the next Godunov launch.
-__global__ void __launch_bounds__(256) k_courant_final(const double *__restrict__ part, int nblocks, double dtmax,
-                                                       double *__restrict__ out, double *__restrict__ dt_out,
-                                                       double *__restrict__ t_acc, int acc_old)
+// courant_fine over octs of several levels: c0 carries the level-independent
+// constants (c0.dx = finest dx, c0.gsum_dx = sum |g|), dx of an oct is
+// boxlen*2^-lev (the same value the host computes as boxlen/2^lev).
+__global__ void __launch_bounds__(kCourantThreads) k_courant_ml(const double *__restrict__ uold,
+                                                                const uint8_t *__restrict__ refined,
+                                                                const int32_t *__restrict__ lev, int head, int noct,
+                                                                const CourantK c0, double boxlen,
+                                                                double *__restrict__ part)
+{
+    double mass = 0, ekin = 0, eint = 0, dtmin = c0.dx / c0.smallc;
+    const long ncell = (long)noct * 8;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += (long)gridDim.x * blockDim.x) {
+        const long o = (long)(head - 1) + i / 8;
+        const int ind = (int)(i % 8);
+        if (refined[o * 8 + ind]) continue;
+        CourantK c = c0;
+        const double dx = __builtin_ldexp(boxlen, -lev[o]);
+        c.dx = dx;
+        c.vol = dx * dx * dx;
+        c.gsum_dx = c0.gsum_dx * dx;
+        const double *p = uold + o * 40 + ind;
+        const double u[5] = {p[0], p[8], p[16], p[24], p[32]};
+        courant_cell(u, c, mass, ekin, eint, dtmin);
+    }
+    __shared__ double red[kCourantThreads / 64][4];
+    mass = wave_sum(mass);
+    ekin = wave_sum(ekin);
+    eint = wave_sum(eint);
+    dtmin = wave_min(dtmin);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) {
+        red[w][0] = mass;
+        red[w][1] = ekin;
+        red[w][2] = eint;
+        red[w][3] = dtmin;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0, s1 = 0, s2 = 0, m = red[0][3];
+        for (int i = 0; i < kCourantThreads / 64; i++) {
+            s0 += red[i][0];
+            s1 += red[i][1];
+            s2 += red[i][2];
+            m = red[i][3] < m ? red[i][3] : m;
+        }
+        part[4 * blockIdx.x + 0] = s0;
+        part[4 * blockIdx.x + 1] = s1;
+        part[4 * blockIdx.x + 2] = s2;
+        part[4 * blockIdx.x + 3] = m;
+    }
+}
+
+__global__ void k_zero(double *p, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// Coarse-fine reflux fold: each ghost oct stands for one parent cell.
+__global__ void k_reflux_fold(double *unew, const double *__restrict__ racc, const int32_t *__restrict__ ghost_parent,
+                              int nghost, int ttd)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nghost) return;
+    const int igrid = ghost_parent[2 * g], icell = ghost_parent[2 * g + 1];
+    if (igrid <= 0) return;
+    double *p = unew + (size_t)(igrid - 1) * 5 * ttd + (icell - 1);
+#pragma unroll
+    for (int n = 0; n < 5; n++) p[n * ttd] = p[n * ttd] + racc[5 * (size_t)g + n];
+}
+
+// Fixed-order reduction of the per-block partials by one 256-thread block:
+// out = {mass, ekin, eint, emag, dt}; dt_out (optional) receives dt for the
+// next Godunov launch; t_acc (optional) accumulates the time (the old dt_out
+// when acc_old, else the new dt).
+__device__ __forceinline__ void courant_final_block(const double *__restrict__ part, int nblocks, double dtmax,
+                                                    double *__restrict__ out, double *__restrict__ dt_out,
+                                                    double *__restrict__ t_acc, int acc_old)
 {
     __shared__ double red[4][4];
     double s0 = 0, s1 = 0, s2 = 0, m = dtmax;
@@ -706,6 +822,20 @@ __global__ void __launch_bounds__(256) k_courant_final(const double *__restrict_
         if (t_acc) *t_acc += (acc_old && dt_out) ? *dt_out : mm;
         if (dt_out) *dt_out = mm;
     }
+}
+
+__global__ void __launch_bounds__(256) k_courant_final(const double *__restrict__ part, int nblocks, double dtmax,
+                                                       double *__restrict__ out, double *__restrict__ dt_out,
+                                                       double *__restrict__ t_acc, int acc_old)
+{
+    courant_final_block(part, nblocks, dtmax, out, dt_out, t_acc, acc_old);
+}
+
+// One block per job of a batch: each job's own partials, in its fixed order.
+__global__ void __launch_bounds__(256) k_courant_final_batch(const CourantJob *__restrict__ jobs)
+{
+    const CourantJob j = jobs[blockIdx.x];
+    courant_final_block(j.part, j.nparts, j.dtmax, j.out, j.dt_inout, j.t_acc, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -897,19 +1027,33 @@ hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, 
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_godunov3d_wl(const GodArgs *descs, const int2 *items, int nitems, int nblocks, int bx, int by,
-                               int bz, int prec, hipStream_t s)
+hipError_t launch_godunov3d_wl(const WorkList &w, int bx, int by, int bz, int slope, int riemann,
+                               int prec, hipStream_t s)
 {
-    if (nitems <= 0) return hipSuccess;
+    if (w.range[8] <= 0) return hipSuccess;
+    int maxr = 0;
+    for (int r = 0; r < 8; r++) maxr = w.range[r + 1] - w.range[r] > maxr ? w.range[r + 1] - w.range[r] : maxr;
+    const int nblocks = 8 * maxr;
 #define RH_LAUNCH_WL(A, B, C)                                                                            \
     if (bx == A && by == B && bz == C) {                                                                 \
         constexpr int nt = (2 * A + 2) * (2 * B + 2);                                                     \
-        if (prec == 0)                                                                                   \
-            hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, double, 0, 3>), dim3(nblocks), dim3(nt), 0, s,   \
-                               descs, items, nitems);                                                    \
-        else                                                                                             \
-            hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, float, 0, 3>), dim3(nblocks), dim3(nt), 0, s,    \
-                               descs, items, nitems);                                                    \
+        const dim3 g(nblocks), b(nt);                                                                    \
+        if (slope == 3) {                                                                                \
+            if (prec == 0)                                                                               \
+                hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, double, 0, 3>), g, b, 0, s, w);              \
+            else                                                                                         \
+                hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, float, 0, 3>), g, b, 0, s, w);               \
+        } else if (riemann == 3) {                                                                       \
+            if (prec == 0)                                                                               \
+                hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, double, 3, 2>), g, b, 0, s, w);              \
+            else                                                                                         \
+                hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, float, 3, 2>), g, b, 0, s, w);               \
+        } else {                                                                                         \
+            if (prec == 0)                                                                               \
+                hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, double, 0, 2>), g, b, 0, s, w);              \
+            else                                                                                         \
+                hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, float, 0, 2>), g, b, 0, s, w);               \
+        }                                                                                                \
         return hipGetLastError();                                                                        \
     }
     RH_BRICK_SHAPES(RH_LAUNCH_WL)
@@ -933,6 +1077,61 @@ hipError_t launch_courant_reduce(const double *partials, int nparts, double dtma
                                  double *t_acc, hipStream_t s)
 {
     hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(256), 0, s, partials, nparts, dtmax, out, dt_inout, t_acc, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_courant_reduce_batch(const CourantJob *jobs, int njobs, hipStream_t s)
+{
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_courant_final_batch, dim3(njobs), dim3(256), 0, s, jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_courant_ml(const double *uold, const uint8_t *refined, const int32_t *lev, int head, int noct,
+                             double boxlen, double dx_finest, double gamma, double smallr, double smallc,
+                             double courant_factor, const double g[3], double *partials, int max_blocks, double *out,
+                             double *dt_out, hipStream_t s, double *t_acc)
+{
+    CourantK c;
+    c.dx = dx_finest;
+    c.vol = 0.0;
+    c.gamma = gamma;
+    c.smallr = smallr;
+    c.smallc = smallc;
+    c.courant_factor = courant_factor;
+    c.smallp = (smallc * smallc) / gamma;
+    double gs = 0.0;
+    for (int d = 0; d < 3; d++) gs = gs + __builtin_fabs(g[d]);
+    c.gsum_dx = gs;  // times dx per oct in the kernel
+    c.ndim = 3;
+    const long ncell = (long)noct * 8;
+    int nb = (int)((ncell + kCourantThreads - 1) / kCourantThreads);
+    nb = nb < max_blocks ? nb : max_blocks;
+    nb = nb < 1 ? 1 : nb;
+    hipLaunchKernelGGL(k_courant_ml, dim3(nb), dim3(kCourantThreads), 0, s, uold, refined, lev, head, noct, c, boxlen,
+                       partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(256), 0, s, partials, nb, dx_finest / smallc, out, dt_out,
+                       t_acc, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(double *p, long n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    long nb = (n + 255) / 256;
+    nb = nb < 4096 ? nb : 4096;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)nb), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_reflux_fold(double *unew, const double *racc, const int32_t *ghost_parent, int nghost, int ttd,
+                              hipStream_t s)
+{
+    if (nghost <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reflux_fold, dim3((nghost + 255) / 256), dim3(256), 0, s, unew, racc, ghost_parent, nghost,
+                       ttd);
     return hipGetLastError();
 }
 

@@ -1,0 +1,313 @@
+// Ensemble serving extensions (include/ramses2_hip_ext.h): contexts that each
+// hold one job's mesh, device-side initial conditions, and batched time steps
+// that advance many jobs with shared launches (one work-list Godunov kernel
+// over the bricks of every job, one per-job Courant reduction).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/ramses2_hip.h"
+#include "../../include/ramses2_hip_ext.h"
+#include "rh_godunov.h"
+#include "rh_state.h"
+
+using namespace rh;
+
+namespace {
+
+int efail(int code, const std::string &msg)
+{
+    set_error(msg);
+    return code;
+}
+
+// Initial conditions of the ensemble jobs, one thread per cell of the level.
+// kind 1, blast: ambient (rho0, 0, p0), pressure pb in the 2^3 cells whose
+//   integer coordinates are all in [lo, lo+2) (the energy E deposited in them,
+//   pb = (gamma-1) E / V computed by the caller);
+// kind 2, planar two-state Riemann problem: (rl, 0, pl) where
+//   (i_axis + 0.5)/ncell < 0.5, else (rr, 0, pr).
+// Conservative variables as prim_to_cons: E = p/(gamma-1) + rho v^2/2.
+__global__ void k_init_ic(double *uold, double *unew, const int32_t *__restrict__ ckey, int head, int noct,
+                          int ncell, int kind, double a0, double a1, double a2, double a3, double a4, double gamma)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)noct * 8) return;
+    const long o = (long)(head - 1) + i / 8;
+    const int ind = (int)(i % 8);
+    const int c[3] = {2 * ckey[3 * o] + (ind & 1), 2 * ckey[3 * o + 1] + ((ind >> 1) & 1),
+                      2 * ckey[3 * o + 2] + ((ind >> 2) & 1)};
+    double rho, p;
+    if (kind == 1) {
+        const int lo = (int)a3;
+        const bool in = c[0] >= lo && c[0] < lo + 2 && c[1] >= lo && c[1] < lo + 2 && c[2] >= lo && c[2] < lo + 2;
+        rho = a0;
+        p = in ? a2 : a1;
+    } else {
+        const int axis = (int)a4;
+        const bool left = ((double)c[axis] + 0.5) / (double)ncell < 0.5;
+        rho = left ? a0 : a2;
+        p = left ? a1 : a3;
+    }
+    const double ekin = 0.5 * rho * (0.0 * 0.0 + 0.0 * 0.0 + 0.0 * 0.0);
+    const double u[5] = {rho, rho * 0.0, rho * 0.0, rho * 0.0, p / (gamma - 1.0) + ekin};
+    double *po = uold + o * 40 + ind;
+    double *pn = unew + o * 40 + ind;
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+        po[8 * n] = u[n];
+        pn[8 * n] = u[n];
+    }
+}
+
+// Device and pinned staging buffers of batched steps (shared by all contexts).
+struct Batch {
+    int cap_jobs = 0, cap_items = 0;
+    GodArgs *d_descs = nullptr;    // [2][cap_jobs]: parity 0 (uold->unew) and 1 (unew->uold)
+    CourantJob *d_cjobs = nullptr; // [cap_jobs]
+    int2 *d_items = nullptr;       // [cap_items]
+    GodArgs *h_descs = nullptr;
+    CourantJob *h_cjobs = nullptr;
+    int2 *h_items = nullptr;
+    hipEvent_t staged = nullptr;   // the last upload from the staging buffers has completed
+    bool pending = false;
+};
+Batch g_batch;
+
+int batch_reserve(int njobs, int nitems, hipStream_t s)
+{
+    Batch &b = g_batch;
+    if (!b.staged) {
+        RH_HIP_CHECK(hipEventCreateWithFlags(&b.staged, hipEventDisableTiming));
+    }
+    if (b.pending) {
+        RH_HIP_CHECK(hipEventSynchronize(b.staged));
+        b.pending = false;
+    }
+    if (njobs > b.cap_jobs) {
+        RH_HIP_CHECK(hipStreamSynchronize(s));
+        (void)hipFree(b.d_descs);
+        (void)hipFree(b.d_cjobs);
+        (void)hipHostFree(b.h_descs);
+        (void)hipHostFree(b.h_cjobs);
+        const int cap = std::max(njobs, 2 * b.cap_jobs);
+        RH_HIP_CHECK(hipMalloc(&b.d_descs, 2 * (size_t)cap * sizeof(GodArgs)));
+        RH_HIP_CHECK(hipMalloc(&b.d_cjobs, (size_t)cap * sizeof(CourantJob)));
+        RH_HIP_CHECK(hipHostMalloc(&b.h_descs, 2 * (size_t)cap * sizeof(GodArgs), hipHostMallocDefault));
+        RH_HIP_CHECK(hipHostMalloc(&b.h_cjobs, (size_t)cap * sizeof(CourantJob), hipHostMallocDefault));
+        b.cap_jobs = cap;
+    }
+    if (nitems > b.cap_items) {
+        RH_HIP_CHECK(hipStreamSynchronize(s));
+        (void)hipFree(b.d_items);
+        (void)hipHostFree(b.h_items);
+        const int cap = std::max(nitems, 2 * b.cap_items);
+        RH_HIP_CHECK(hipMalloc(&b.d_items, (size_t)cap * sizeof(int2)));
+        RH_HIP_CHECK(hipHostMalloc(&b.h_items, (size_t)cap * sizeof(int2), hipHostMallocDefault));
+        b.cap_items = cap;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rh_ctx_create(int32_t *id)
+{
+    if (!id) return efail(RH_ERR_ARG, "null id");
+    State *s = new State();
+    *id = ctx_add(s);
+    return 0;
+}
+
+int rh_ctx_select(int id)
+{
+    State *s = ctx_get(id);
+    if (!s) return efail(RH_ERR_ARG, "no context with id " + std::to_string(id));
+    ctx_select(s);
+    return 0;
+}
+
+int rh_ctx_current(int32_t *id)
+{
+    if (!id) return efail(RH_ERR_ARG, "null id");
+    State *cur = &st();
+    for (int i = 0; i < ctx_count(); i++)
+        if (ctx_get(i) == cur) {
+            *id = i;
+            return 0;
+        }
+    return efail(RH_ERR_STATE, "current context not in the table");
+}
+
+int rh_ctx_destroy(int id)
+{
+    State *s = ctx_get(id);
+    if (!s) return efail(RH_ERR_ARG, "no context with id " + std::to_string(id));
+    State *prev = &st();
+    ctx_select(s);
+    (void)rh_finalize();
+    if (id == 0) {
+        ctx_select(prev);
+        return 0;
+    }
+    ctx_set_slot(id, nullptr);
+    delete s;
+    ctx_select(prev == s ? ctx_get(0) : prev);
+    return 0;
+}
+
+int rh_ctx_footprint(int id, int64_t *bytes)
+{
+    State *s = ctx_get(id);
+    if (!s) return efail(RH_ERR_ARG, "no context with id " + std::to_string(id));
+    if (bytes) *bytes = s->dev_bytes;
+    return 0;
+}
+
+int rh_estimate_uniform_bytes(int n_octs_per_dim, int ncachemax, int hash_size, int64_t *bytes)
+{
+    if (n_octs_per_dim < 1 || ncachemax < 0 || hash_size < 1 || !bytes)
+        return efail(RH_ERR_ARG, "rh_estimate_uniform_bytes: bad arguments");
+    const int64_t n = n_octs_per_dim, noct = n * n * n, ntot = noct + ncachemax;
+    int64_t cap = 1;
+    while (cap < hash_size) cap <<= 1;
+    const int bx = st().brick_bx, by = st().brick_by, bz = st().brick_bz;
+    const int64_t nb = ((n + bx - 1) / bx) * ((n + by - 1) / by) * ((n + bz - 1) / bz);
+    int64_t b = 0;
+    b += 2 * ntot * 40 * 8;                                   // uold, unew
+    b += ntot * (12 + 4 + 8);                                 // ckey, lev, refined
+    b += cap * (8 + 4);                                       // hash keys, values
+    b += (int64_t)std::max(ncachemax, 1) * (2 + 14 + 21 + 3) * 4;  // ghost-oct tables
+    b += 8 * 8 + 2 * 8 + 4 * 4 + 2 * 8;                       // reduction / dt / counters
+    b += std::max<int64_t>(4096, nb) * 4 * 8;                 // Courant partials
+    b += noct * 27 * 4 + noct * 2 * 4;                        // neighbour and father tables
+    b += nb * ((int64_t)(bx + 2) * (by + 2) * (bz + 2) + 3) * 4;  // brick windows and origins
+    *bytes = b;
+    return 0;
+}
+
+int rh_init_ic(int head_idx, int num_octs, int ncell, int kind, const double params[5], double gamma)
+{
+    State &s = st();
+    if (!s.inited || !s.uold) return efail(RH_ERR_STATE, "library not initialised / mesh not allocated");
+    if (s.ndim != 3) return efail(RH_ERR_ARG, "rh_init_ic builds 3D initial conditions");
+    if (kind != 1 && kind != 2) return efail(RH_ERR_ARG, "kind must be 1 (blast) or 2 (Riemann)");
+    if (!params || ncell < 2 || head_idx < 1 || num_octs < 0 || head_idx - 1 + num_octs > s.ngridtot)
+        return efail(RH_ERR_ARG, "rh_init_ic: bad oct range or parameters");
+    if (kind == 2 && (params[4] < 0 || params[4] > 2)) return efail(RH_ERR_ARG, "axis must be 0, 1 or 2");
+    const long n = (long)num_octs * 8;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_init_ic, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s.stream, s.uold, s.unew, s.ckey,
+                       head_idx, num_octs, ncell, kind, params[0], params[1], params[2], params[3], params[4], gamma);
+    RH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int rh_job_prepare(int head_idx, int num_octs, int ilevel, double dx, double gamma, double smallr, double smallc2,
+                   double courant_factor, const double g[3], int slope, int riemann)
+{
+    State &s = st();
+    if (!s.inited || !s.uold) return efail(RH_ERR_STATE, "library not initialised / mesh not allocated");
+    if (s.ndim != 3) return efail(RH_ERR_ARG, "batched jobs are 3D");
+    GodArgs a;
+    const LevelBricks *lb = nullptr;
+    if (int r = fill_args3(head_idx, num_octs, ilevel, dx, gamma, smallr, smallc2, courant_factor, g, slope,
+                           riemann, s.dt_dev, 0.0, a, lb))
+        return r;
+    if (int r = ensure_partials(lb->nbricks)) return r;
+    a.courant = 1;
+    a.cpart = s.red_part;
+    s.job = a;
+    s.job_bx = lb->bx;
+    s.job_by = lb->by;
+    s.job_bz = lb->bz;
+    s.job_dtmax = dx / __builtin_sqrt(smallc2);
+    // dt of the first step: a standalone Courant pass over the initial state
+    const double gz[3] = {0, 0, 0};
+    RH_HIP_CHECK(hipMemsetAsync(s.t_dev, 0, sizeof(double), s.stream));
+    RH_HIP_CHECK(launch_courant(s.uold, s.refined, head_idx, num_octs, 3, dx, gamma, smallr, __builtin_sqrt(smallc2),
+                                courant_factor, g ? g : gz, s.red_part, s.red_cap, s.red_out, s.dt_dev, s.stream,
+                                nullptr));
+    s.job_ready = true;
+    return 0;
+}
+
+int rh_batch_step(const int32_t *ids, int njobs, int precision, int nsteps)
+{
+    if (njobs < 0 || nsteps < 0 || (njobs > 0 && !ids)) return efail(RH_ERR_ARG, "rh_batch_step: bad arguments");
+    if (precision != 0 && precision != 1) return efail(RH_ERR_ARG, "precision must be 0 or 1");
+    if (njobs == 0 || nsteps == 0) return 0;
+    std::vector<State *> js(njobs);
+    long nitems = 0;
+    for (int j = 0; j < njobs; j++) {
+        State *s = ctx_get(ids[j]);
+        if (!s || !s->job_ready)
+            return efail(RH_ERR_STATE, "context " + std::to_string(ids[j]) + " has no prepared job");
+        for (int k = 0; k < j; k++)
+            if (js[k] == s) return efail(RH_ERR_ARG, "a context appears twice in the batch");
+        js[j] = s;
+        if (s->job_bx != js[0]->job_bx || s->job_by != js[0]->job_by || s->job_bz != js[0]->job_bz ||
+            s->job.slope != js[0]->job.slope || s->job.riemann != js[0]->job.riemann)
+            return efail(RH_ERR_ARG, "the jobs of one batch need the same brick shape, slope and Riemann solver");
+        nitems += s->job.nbricks;
+    }
+    if (nitems > (1l << 30)) return efail(RH_ERR_ARG, "too many bricks in one batch");
+    hipStream_t stream = js[0]->stream;
+    if (int r = batch_reserve(njobs, (int)nitems, stream)) return r;
+    Batch &b = g_batch;
+    int pos = 0;
+    for (int j = 0; j < njobs; j++) {
+        State &s = *js[j];
+        GodArgs a = s.job;
+        a.uold = s.uold;
+        a.unew = s.unew;
+        a.ubase = s.uold;
+        b.h_descs[j] = a;
+        a.uold = s.unew;
+        a.unew = s.uold;
+        a.ubase = s.unew;
+        b.h_descs[njobs + j] = a;
+        b.h_cjobs[j] = CourantJob{s.red_part, s.job.nbricks, s.job_dtmax, s.red_out, s.dt_dev, s.t_dev};
+        for (int i = 0; i < s.job.nbricks; i++) b.h_items[pos++] = make_int2(j, i);
+    }
+    RH_HIP_CHECK(hipMemcpyAsync(b.d_descs, b.h_descs, 2 * (size_t)njobs * sizeof(GodArgs), hipMemcpyHostToDevice,
+                                stream));
+    RH_HIP_CHECK(hipMemcpyAsync(b.d_cjobs, b.h_cjobs, (size_t)njobs * sizeof(CourantJob), hipMemcpyHostToDevice,
+                                stream));
+    RH_HIP_CHECK(hipMemcpyAsync(b.d_items, b.h_items, (size_t)nitems * sizeof(int2), hipMemcpyHostToDevice, stream));
+    RH_HIP_CHECK(hipEventRecord(b.staged, stream));
+    b.pending = true;
+    WorkList w;
+    w.items = b.d_items;
+    for (int r = 0; r <= 8; r++) w.range[r] = (int)((nitems * r) / 8);
+    for (int k = 0; k < nsteps; k++) {
+        w.descs = b.d_descs + (k & 1) * njobs;
+        RH_HIP_CHECK(launch_godunov3d_wl(w, js[0]->job_bx, js[0]->job_by, js[0]->job_bz, js[0]->job.slope,
+                                         js[0]->job.riemann, precision, stream));
+        RH_HIP_CHECK(launch_courant_reduce_batch(b.d_cjobs, njobs, stream));
+    }
+    if (nsteps & 1)
+        for (State *s : js) std::swap(s->uold, s->unew);
+    return 0;
+}
+
+int rh_job_status(int id, double out[7])
+{
+    State *s = ctx_get(id);
+    if (!s || !s->inited) return efail(RH_ERR_ARG, "no initialised context with id " + std::to_string(id));
+    if (!out) return efail(RH_ERR_ARG, "null output");
+    double v[8];
+    RH_HIP_CHECK(hipMemcpyAsync(v, s->t_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    RH_HIP_CHECK(hipMemcpyAsync(v + 1, s->dt_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    RH_HIP_CHECK(hipMemcpyAsync(v + 2, s->red_out, 5 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    RH_HIP_CHECK(hipStreamSynchronize(s->stream));
+    for (int i = 0; i < 7; i++) out[i] = v[i];
+    return 0;
+}
+
+}  // extern "C"

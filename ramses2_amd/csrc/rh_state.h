@@ -7,6 +7,9 @@
 #include <string>
 #include <vector>
 
+#include "../../include/ramses2_hip.h"
+#include "rh_godunov.h"
+
 namespace rh {
 
 // Status of the last call, reported through rh_last_error().
@@ -50,6 +53,25 @@ struct LevelNbor {
     int boxmax[3] = {0, 0, 0};
 };
 
+// Batched multi-level stepping of an AMR mesh (rh_amr_prepare / rh_amr_run):
+// one work-list Godunov launch over the bricks of every level per step.
+struct AmrPlan {
+    bool ready = false;
+    int levelmin = 0, levelmax = 0;
+    int head0 = 0, noct_all = 0;   // the levels' octs form one contiguous range
+    double boxlen = 1.0, dx_finest = 0.0;
+    double gamma = 1.4, smallr = 0.0, smallc2 = 0.0, courant = 0.0, g[3] = {0, 0, 0};
+    int slope = 0, riemann = 0;
+    int bx = 0, by = 0, bz = 0;
+    int ndesc = 0, nitems = 0;
+    GodArgs *d_descs = nullptr;    // [2][ndesc]: arrays in both orders
+    std::vector<GodArgs> h_descs;  // host copy of d_descs
+    int2 *d_items = nullptr;       // [nitems] (level descriptor, brick)
+    int range[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double *racc = nullptr;        // [max(nghost_total,1)][5] reflux sums per ghost oct
+    std::vector<int> levels;       // levels with octs, coarse to fine
+};
+
 struct State {
     bool inited = false;
     int device = 0;
@@ -78,6 +100,9 @@ struct State {
     int brick_bx = 7, brick_by = 7, brick_bz = 8;
     // Godunov precision: 0 FP64, 1 FP32 reconstruction/fluxes + FP64 accumulation
     int precision = 0;
+    // Courant pass of the next step fused into the Godunov kernel's epilogue:
+    // -1 automatic (fused for the mixed precision kernel only), 0 off, 1 on
+    int fused_courant = -1;
     // timer events
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // single-level device time stepping: dt and accumulated time on the device
@@ -97,8 +122,40 @@ struct State {
     double *red_out = nullptr;     // device, 8 doubles
     double *h_red = nullptr;       // pinned host, 8 doubles
     std::string last_error;
+    // device allocations of this context (bytes by pointer) and their total
+    std::map<void *, size_t> allocs;
+    int64_t dev_bytes = 0;
+    // a job prepared for batched stepping (one 3D level, dt on the device)
+    bool job_ready = false;
+    GodArgs job;
+    int job_bx = 0, job_by = 0, job_bz = 0;
+    double job_dtmax = 0.0;
+    AmrPlan amr;
 };
 
+// Godunov arguments of a 3D level of the current context (fused mode,
+// unew = uold + dF); lb receives the level's bricks.  Returns an RH_ERR code.
+int fill_args3(int head_idx, int num_octs, int ilevel, double dx, double gamma, double smallr, double smallc2,
+               double courant_factor, const double g[3], int slope, int riemann, const double *dt_dev, double dt,
+               GodArgs &a, const LevelBricks *&lb);
+int fill_args1(int head_idx, int num_octs, int ilevel, double dx, double gamma, double smallr, double smallc2,
+               double courant_factor, const double g[3], int slope, int riemann, const double *dt_dev, double dt,
+               God1Args &a);
+// Grow the current context's per-brick partial-sum buffer to n entries.
+int ensure_partials(int n);
+
+// The current context (the drop-in entry points act on it).
 State &st();
+
+// Device allocation / release accounted to the current context.
+hipError_t dmalloc(void **p, size_t n);
+void dfree(void *p);
+
+// Context table: id 0 is the default context, always present.
+int ctx_count();
+State *ctx_get(int id);
+int ctx_add(State *s);
+void ctx_set_slot(int id, State *s);
+void ctx_select(State *s);
 
 }  // namespace rh

@@ -193,12 +193,15 @@ class UniformLevel:
         return dt
 
     def run_device(self, nsteps, dt_fixed=0.0, precision=0, use_graph=False, gravity=(0.0, 0.0, 0.0),
-                   time_kernel=False):
+                   time_kernel=False, fused_courant=-1):
         """nsteps device-resident steps (dt reduced on the device).  Returns
         (dt_last, t_total, sums, godunov_ms) where godunov_ms is the mean
-        Godunov launch duration (HIP events) when time_kernel is set."""
+        Godunov launch duration (HIP events) when time_kernel is set.
+        fused_courant: -1 automatic, 0 separate Courant kernel, 1 Courant
+        reduction in the Godunov kernel's epilogue."""
         g = np.ascontiguousarray(gravity, dtype=np.float64)
         call("rh_set_precision", int(precision))
+        call("rh_set_fused_courant", int(fused_courant))
         dt_last = ctypes.c_double()
         t_tot = ctypes.c_double()
         sums = np.zeros(4)
@@ -275,6 +278,27 @@ class AmrHierarchy:
         self.t += dt
         self.nstep += 1
         return dt
+
+    def run_device(self, nsteps, precision=None, use_graph=False, gravity=(0.0, 0.0, 0.0)):
+        """nsteps batched steps of all levels (rh_amr_prepare + rh_amr_run): one
+        work-list Godunov launch over every level's bricks per step, dt kept
+        on the device.  Returns (dt_next, t_advanced, sums, ms_per_step)."""
+        am = self.am
+        prec = self.precision if precision is None else precision
+        head, noct = am.heads()
+        g = np.ascontiguousarray(gravity, dtype=np.float64)
+        call("rh_amr_prepare", am.levelmin, am.levelmax, ptr(head, _lib._ip), ptr(noct, _lib._ip), am.boxlen,
+             self.p.gamma, self.p.smallr, self.p.smallc ** 2, self.p.courant_factor, ptr(g), self.p.slope_type,
+             self.p.riemann_id)
+        dt = ctypes.c_double()
+        t = ctypes.c_double()
+        sums = np.zeros(4)
+        ms = ctypes.c_float()
+        call("rh_amr_run", int(nsteps), int(prec), int(use_graph), ctypes.byref(dt), ctypes.byref(t), ptr(sums),
+             ctypes.byref(ms))
+        self.t += t.value
+        self.nstep += nsteps
+        return dt.value, t.value, sums, ms.value
 
     def state(self):
         return self.hh.transfer_grid_host(self.am.ngrid)

@@ -51,14 +51,21 @@ def level_params(nlevelmax, ndim, box_octs_at=None):
 
 
 class UniformMesh:
-    """One uniform level of 2^(level-1) octs per dimension, octs in Morton order."""
+    """One uniform level of n octs per dimension, octs in Morton order.
 
-    def __init__(self, ndim, level, boxlen=1.0):
+    n defaults to the full level, 2^(level-1); a smaller n is a computational
+    box [0, n) per dimension inside the level (box_ckey_min = 0,
+    box_ckey_max = n, amr/init_amr.f90:250-280), which gives grids of any
+    even number of cells per dimension."""
+
+    def __init__(self, ndim, level, boxlen=1.0, n=None):
         assert ndim in (1, 3)
         self.ndim = ndim
         self.level = level
         self.boxlen = boxlen
-        self.n = 2 ** (level - 1)             # octs per dimension
+        self.n = 2 ** (level - 1) if n is None else int(n)   # octs per dimension
+        assert 1 <= self.n <= 2 ** (level - 1)
+        self.full_box = self.n == 2 ** (level - 1)
         self.ncell = 2 * self.n               # cells per dimension
         self.ttd = 2 ** ndim
         self.dx = boxlen / 2 ** level
@@ -122,7 +129,31 @@ class UniformMesh:
         return c.reshape(nv, self.ncell, self.ncell, self.ncell)
 
     def level_params(self, nlevelmax=None):
-        return level_params(nlevelmax or self.level, self.ndim)
+        ckey_max, key_off, box_min, box_max = level_params(nlevelmax or self.level, self.ndim)
+        if not self.full_box:
+            for l in range(1, len(ckey_max) + 1):
+                sh = l - self.level
+                b = self.n << sh if sh >= 0 else max(1, self.n >> -sh)
+                box_max[l - 1, :self.ndim] = b
+        return ckey_max, key_off, box_min, box_max
+
+    def box(self):
+        """(lev, ckey_min, ckey_max) of the computational box, or None for the full level."""
+        if self.full_box:
+            return None
+        hi = [self.n if d < self.ndim else 1 for d in range(3)]
+        return (self.level, [0, 0, 0], hi)
+
+    @classmethod
+    def for_cells(cls, ndim, ncell, boxlen=1.0):
+        """The smallest level holding ncell (even) cells per dimension, with the
+        box length scaled so that the box of ncell cells has size boxlen."""
+        assert ncell % 2 == 0 and ncell >= 2
+        n = ncell // 2
+        level = 1
+        while 2 ** (level - 1) < n:
+            level += 1
+        return cls(ndim, level, boxlen=boxlen * 2 ** level / ncell, n=n)
 
 
 def prim_to_cons(rho, vel, p, gamma):

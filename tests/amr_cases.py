@@ -19,6 +19,20 @@ def two_level_mesh(lo=4, region=(4, 12), offset=(0, 0, 0)):
     return AmrMesh(3, {lo: k4, lo + 1: k5})
 
 
+def three_level_mesh(lo=4):
+    """Complete level lo, a block of level lo+1 octs (keys [4,12)^3) and inside
+    it a block of level lo+2 octs (keys [12,20)^3): ghost octs of both fine
+    levels have their parent cells and the parents' neighbours."""
+    n = 2 ** (lo - 1)
+    ii = np.arange(n)
+    k4 = np.stack(np.meshgrid(ii, ii, ii, indexing="ij"), -1).reshape(-1, 3)
+    jj = np.arange(4, 12)
+    k5 = np.stack(np.meshgrid(jj, jj, jj, indexing="ij"), -1).reshape(-1, 3)
+    kk = np.arange(12, 20)
+    k6 = np.stack(np.meshgrid(kk, kk, kk, indexing="ij"), -1).reshape(-1, 3)
+    return AmrMesh(3, {lo: k4, lo + 1: k5, lo + 2: k6})
+
+
 def smooth_state(am, seed, gamma=1.4, amp=0.3, noise=0.02, blast=False, shift_x=0.0):
     rng = np.random.default_rng(seed)
     u = np.zeros((am.ngrid, 5, 8))

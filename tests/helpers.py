@@ -16,7 +16,7 @@ def oracle_uniform(mesh: UniformMesh, uold, periodic=True, bound_type=2, unew=No
     bt = ((bound_type, bound_type),) * 3
     return O.OracleMesh(mesh.ndim, mesh.ckey, np.full(mesh.noct, nl), np.zeros((mesh.noct, 2 ** mesh.ndim)),
                         uold, head, noct, levelmin=nl, boxlen=mesh.boxlen, periodic=per, bound_type=bt,
-                        unew=unew)
+                        unew=unew, box=mesh.box())
 
 
 def oracle_params(p):

@@ -10,7 +10,8 @@ import pytest
 
 from ramses2_amd import HydroParams
 from ramses2_amd.hydro import AmrHierarchy
-from tests.amr_cases import leaf_total, oracle_mesh, oracle_step, restrict_all, smooth_state, two_level_mesh
+from tests.amr_cases import (leaf_total, oracle_mesh, oracle_step, restrict_all, smooth_state, three_level_mesh,
+                             two_level_mesh)
 from tests.helpers import rel_err_per_var
 
 pytestmark = pytest.mark.gpu
@@ -59,5 +60,47 @@ def test_two_level_mixed_precision():
     gpu = AmrHierarchy(am, om.uold.copy(), p, interpol_var=0, interpol_type=2, precision=1)
     dt = oracle_step(am, om, p)
     gpu.step(dt=dt)
+    err = rel_err_per_var(gpu.state(), om.uold)
+    assert np.all(err < 2e-6), err
+
+
+@pytest.mark.parametrize("mesh", ["two", "three"])
+@pytest.mark.parametrize("graph", [False, True])
+def test_batched_levels_match_oracle(mesh, graph):
+    """rh_amr_run: every level in one work-list launch per step, refluxes
+    folded after the launch, restriction and the Courant pass on the device,
+    eager or replayed as a hipGraph, against the oracle's amr_step."""
+    am = two_level_mesh() if mesh == "two" else three_level_mesh()
+    u = smooth_state(am, 6, blast=True)
+    om = oracle_mesh(am, u, 0, 2)
+    restrict_all(am, om)
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
+    print("case", mesh, graph, flush=True)
+    gpu = AmrHierarchy(am, om.uold.copy(), p, interpol_var=0, interpol_type=2)
+    t = 0.0
+    for _ in range(4):
+        t += oracle_step(am, om, p)
+    dt, tt, sums, ms = gpu.run_device(4, use_graph=graph)
+    err = rel_err_per_var(gpu.state(), om.uold)
+    assert np.all(err < 1e-12), err
+    assert abs(tt - t) <= 1e-13 * t
+    mass = leaf_total(am, om.uold, 0)
+    assert abs(sums[0] - mass) <= 1e-12 * abs(mass)
+    # a fifth step from where the run stopped (odd count: the arrays swap back)
+    dt_o = oracle_step(am, om, p)
+    assert abs(dt - dt_o) <= 1e-14 * dt_o
+    gpu.run_device(1)
+    assert np.all(rel_err_per_var(gpu.state(), om.uold) < 1e-12)
+
+
+def test_batched_levels_mixed_precision():
+    am = three_level_mesh()
+    u = smooth_state(am, 9, blast=True)
+    om = oracle_mesh(am, u, 0, 2)
+    restrict_all(am, om)
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
+    gpu = AmrHierarchy(am, om.uold.copy(), p, interpol_var=0, interpol_type=2)
+    oracle_step(am, om, p)
+    gpu.run_device(1, precision=1)
     err = rel_err_per_var(gpu.state(), om.uold)
     assert np.all(err < 2e-6), err

@@ -152,9 +152,11 @@ def test_mixed_precision_one_step():
     assert np.all(rel_err_per_var(g, o) < TOLMIX)
 
 
-def test_device_run_matches_dropin_calls():
-    """rh_level_run (dt on the device, set_unew/set_uold folded, hipGraph) gives the
-    same state as the drop-in call sequence, bit for bit."""
+@pytest.mark.parametrize("fused", [0, 1])
+def test_device_run_matches_dropin_calls(fused):
+    """rh_level_run (dt on the device, set_unew/set_uold folded, hipGraph, the
+    Courant pass separate or in the Godunov epilogue) gives the same state as
+    the drop-in call sequence, bit for bit (dt is a min, exact in any order)."""
     level = 4
     mesh = UniformMesh(3, level)
     p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
@@ -164,10 +166,10 @@ def test_device_run_matches_dropin_calls():
         a.step()
     ua = a.state()
     b = UniformLevel(mesh, cells, p)
-    b.run_device(6, use_graph=False, time_kernel=True)
+    b.run_device(6, use_graph=False, time_kernel=True, fused_courant=fused)
     assert np.array_equal(b.state(), ua)
     c = UniformLevel(mesh, cells, p)
-    c.run_device(6, use_graph=True)
+    c.run_device(6, use_graph=True, fused_courant=fused)
     assert np.array_equal(c.state(), ua)
     assert abs(c.t - a.t) <= 1e-15 * a.t
 
