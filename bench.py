@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--amr-steps", type=int, default=20)
+    ap.add_argument("--amr-levels", type=str, default="6,10", help="levelmin,levelmax of the AMR Sedov case")
+    ap.add_argument("--no-amr", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -126,6 +129,75 @@ def run_gpu(args, p, cells, mesh, precision):
     return lev, el, kms, dt_last, t_tot, sums
 
 
+def run_amr(args):
+    """AMR Sedov (levels levelmin..levelmax refined on density/pressure
+    gradients > 0.1, MonCen, HLLC, C 0.5): the batched step (every level in one
+    work-list launch, hipGraph-replayed, brick shape from the autotuner) in FP64
+    and mixed precision, beside the per-level drop-in call sequence."""
+    from ramses2_amd import HydroParams
+    from ramses2_amd.amr_build import sedov_amr
+    from ramses2_amd.autotune import tune_amr
+    from ramses2_amd.hydro import AmrHierarchy, HipHydro
+
+    lmin, lmax = (int(x) for x in args.amr_levels.split(","))
+    t0 = time.perf_counter()
+    am, u0 = sedov_amr(levelmin=lmin, levelmax=lmax, radius=0.3)
+    t_build = time.perf_counter() - t0
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
+    cells = int(sum(am.noct[l] for l in am.levels) * 8)
+    print("bench: AMR Sedov levels %d-%d, octs per level %s, %d cells (mesh built in %.1f s)"
+          % (lmin, lmax, {l: am.noct[l] for l in am.levels}, cells, t_build), file=sys.stderr, flush=True)
+    best, times = tune_amr(am, u0, p, precision=0, cache=None, key="sedov-amr")
+    print("bench: AMR brick shapes (ms/step): %s -> %s" % ({"%dx%dx%d" % k: round(v, 3) for k, v in times.items()},
+                                                           best), file=sys.stderr, flush=True)
+    out = {"workload": "3D Sedov AMR, levels %d-%d, refinement on relative density/pressure gradient > 0.1 "
+                       "(hydro_refine criterion on the analytic blast of radius 0.3), gamma 1.4, MonCen, HLLC, "
+                       "Courant 0.5, periodic, nsubcycle=1" % (lmin, lmax),
+           "octs_per_level": {str(l): int(am.noct[l]) for l in am.levels}, "cells": cells,
+           "brick_shape": "%dx%dx%d" % best,
+           "brick_shape_ms_per_step": {"%dx%dx%d" % k: v for k, v in times.items()}}
+    states = {}
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        HipHydro.set_brick_shape(*best)
+        h = AmrHierarchy(am, u0, p)
+        h.run_device(2, precision=prec, use_graph=True)
+        dt, tt, sums, ms = h.run_device(args.amr_steps, precision=prec, use_graph=True)
+        states[name] = (h.state(), sums)
+        out[name] = {"ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3), "steps": args.amr_steps}
+        print("bench: AMR %s %.3f ms/step (%.3g cell-updates/s)" % (name, ms, cells / (ms * 1e-3)),
+              file=sys.stderr, flush=True)
+    out["mixed"]["speedup_vs_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    u64, s64 = states["fp64"]
+    um, sm = states["mixed"]
+    leaf = am.refined == 0
+    l1 = {}
+    for v, nm in ((0, "rho"), (4, "E")):
+        a, b = u64[:, v][leaf], um[:, v][leaf]
+        l1[nm] = float(np.abs(a - b).sum() / np.abs(a).sum())
+    out["mixed"]["l1_rel_vs_fp64_leaf"] = l1
+    out["mixed"]["mass_drift_vs_fp64"] = float((sm[0] - s64[0]) / s64[0])
+    # the drop-in sequence: per level courant (synchronous), set_unew, godunov, set_uold, upload
+    HipHydro.set_brick_shape(*best)
+    h = AmrHierarchy(am, u0, p)
+    h.step()
+    from ramses2_amd import _lib
+    _lib.call("rh_device_sync")
+    t0 = time.perf_counter()
+    nd = max(3, args.amr_steps // 4)
+    for _ in range(nd):
+        h.step()
+    _lib.call("rh_device_sync")
+    ms_d = (time.perf_counter() - t0) / nd * 1e3
+    out["dropin_per_level_fp64"] = {"ms_per_step": ms_d, "cell_updates_per_s": cells / (ms_d * 1e-3),
+                                    "steps": nd,
+                                    "note": "reference call order per level through the drop-in C-ABI"}
+    out["fp64"]["speedup_vs_dropin"] = ms_d / out["fp64"]["ms_per_step"]
+    print("bench: AMR drop-in per level %.3f ms/step; batched FP64 x%.2f, mixed x%.2f vs FP64"
+          % (ms_d, out["fp64"]["speedup_vs_dropin"], out["mixed"]["speedup_vs_fp64"]), file=sys.stderr, flush=True)
+    HipHydro.set_brick_shape(7, 7, 8)
+    return out
+
+
 def run_serving(args):
     """Ensemble serving: capacity of the batched FP64 step measured on a closed
     batch of the stream's first jobs, then the Poisson stream of all jobs at
@@ -139,6 +211,12 @@ def run_serving(args):
     # (admission timed apart: it is host work between ticks)
     cal = base[:args.serve_slots]
     eng = HipEngine(precision=0)
+    # one template context per grid size of the stream (the server's start-up)
+    t0 = time.perf_counter()
+    for n in sorted({s.ncell for s in base}):
+        eng.warm(n)
+    eng.sync()
+    t_warm = time.perf_counter() - t0
     t0 = time.perf_counter()
     hs = [eng.open(s) for s in cal]
     eng.sync()
@@ -166,11 +244,12 @@ def run_serving(args):
     out = {"jobs": len(base), "arrival_rate_jobs_per_s": rate, "offered_load": args.serve_load,
            "capacity_cell_updates_per_s": capacity, "capacity_note": "%d jobs of the stream resident, %d batched "
            "FP64 steps timed" % (len(cal), ncal), "admission_ms_per_job": t_open * 1e3,
-           "release_ms_per_job": t_close * 1e3, "mean_job_cells": mean_cells, "steps_per_job": args.serve_steps,
+           "release_ms_per_job": t_close * 1e3, "template_build_s": t_warm,
+           "template_bytes": eng.template_bytes(), "mean_job_cells": mean_cells, "steps_per_job": args.serve_steps,
            "max_slots": args.serve_slots, "hbm_budget_bytes": budget,
            "sizes": "N^3, N log-uniform in [32,128], even; 50% blast, 50% Riemann; gamma 1.4, MonCen, HLLC, C 0.5"}
     for prec, name in ((0, "fp64"), (1, "mixed")):
-        eng = HipEngine(precision=prec)
+        eng.precision = prec
         srv = EnsembleServer(eng, max_slots=args.serve_slots, hbm_budget=budget)
         res = srv.run(base)
         summ = latency_summary(res, base)
@@ -181,6 +260,7 @@ def run_serving(args):
         print("bench: serving %s p50 %.3f s p99 %.3f s, %.3g cell-updates/s over %.1f s"
               % (name, summ["p50_s"], summ["p99_s"], summ["throughput_cell_updates_per_s"], summ["span_s"]),
               file=sys.stderr, flush=True)
+    eng.close_all()
     _lib.call("rh_ctx_select", 0)
     return out
 
@@ -245,6 +325,10 @@ def main():
                 "kernel": "k_godunov3d<7,7,8,double> (FP64)", "kernel_ms": kms64,
                 "bytes_per_launch": BYTES_PER_CELL * ncells}
 
+    amr = None
+    if not args.no_amr:
+        amr = run_amr(args)
+
     serving = None
     if not args.no_serve:
         serving = run_serving(args)
@@ -274,6 +358,7 @@ def main():
         "mixed": mixed,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "amr": amr,
         "serving": serving,
     }
     print(json.dumps(out))

@@ -91,6 +91,17 @@ int rh_ctx_current(int32_t *id);
 int rh_ctx_destroy(int id);              /* free its device memory; id 0 is reset   */
 int rh_ctx_footprint(int id, int64_t *bytes);   /* device bytes the context holds  */
 
+/* Give the current (empty) context a copy of the mesh of context src_id: level
+ * parameters, oct records, device hash, neighbour, ghost and brick tables, by
+ * device-to-device copies on the library stream; uold/unew are zeroed.  Serving
+ * builds one template context per grid size and clones it per job. */
+int rh_ctx_clone_mesh(int src_id);
+
+/* Device blocks released by destroyed contexts are kept for reuse by requests
+ * of the same size (no device synchronisation on release).  This hands them
+ * back to the device and reports how many bytes were cached. */
+int rh_cache_release(int64_t *bytes_released);
+
 /* Device bytes a context needs for a uniform 3D level of n^3 octs (n octs per
  * dimension) with the current brick shape: the admission estimate of the
  * serving loop (rh_ctx_footprint reports the bytes actually held). */

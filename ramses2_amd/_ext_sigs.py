@@ -24,6 +24,8 @@ SIGNATURES = {
     "rh_ctx_current": [_ip],
     "rh_ctx_destroy": [_c],
     "rh_ctx_footprint": [_c, ctypes.POINTER(ctypes.c_int64)],
+    "rh_ctx_clone_mesh": [_c],
+    "rh_cache_release": [ctypes.POINTER(ctypes.c_int64)],
     "rh_estimate_uniform_bytes": [_c, _c, _c, ctypes.POINTER(ctypes.c_int64)],
     "rh_init_ic": [_c, _c, _c, _c, _dp, _d],
     "rh_job_prepare": [_c, _c, _c, _d, _d, _d, _d, _d, _dp, _c, _c],

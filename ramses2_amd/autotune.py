@@ -1,0 +1,112 @@
+"""Brick-shape autotuner.
+
+The Godunov kernels process octs in bricks of BX x BY x BZ octs (one
+workgroup per brick, (2BX+2)(2BY+2) threads marching in z).  The best shape
+depends on the mesh (uniform box, sparse AMR shells, small serving jobs), the
+precision and the device, so it is measured: each compiled shape is timed on
+the actual mesh with HIP events, and the fastest is kept.  Results are cached
+per (workload key, precision) in a JSON file next to the library unless a
+path is given.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import call
+
+SHAPES = [(7, 7, 8), (7, 7, 16), (15, 7, 8), (15, 7, 16), (7, 7, 4), (3, 3, 8), (3, 3, 4)]
+DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "autotune.json")
+
+
+def compiled_shapes():
+    """The shapes of SHAPES that this build of the library has kernels for."""
+    out = []
+    for s in SHAPES:
+        v = [ctypes.c_int32() for _ in range(3)]
+        call("rh_get_brick_shape", *[ctypes.byref(x) for x in v])
+        old = tuple(x.value for x in v)
+        ok = _lib.load().rh_set_brick_shape(*s) == 0
+        call("rh_set_brick_shape", *old)
+        if ok:
+            out.append(s)
+    return out
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(path, d):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(d, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def tune(key, precision, run, shapes=None, cache=DEFAULT_CACHE, refresh=False):
+    """Time run(shape) -> ms for each shape (run builds the mesh tables with
+    that shape and returns the mean step time); return (best_shape, {shape: ms}).
+    A cached choice for (key, precision) is returned without timing unless
+    refresh is set."""
+    ck = "%s|prec%d" % (key, precision)
+    db = _load(cache) if cache else {}
+    if not refresh and ck in db:
+        return tuple(db[ck]["best"]), {tuple(map(int, k.split("x"))): v for k, v in db[ck]["ms"].items()}
+    shapes = shapes or compiled_shapes()
+    times = {}
+    for s in shapes:
+        times[s] = float(run(s))
+    best = min(times, key=times.get)
+    if cache:
+        db = _load(cache)
+        db[ck] = {"best": list(best), "ms": {"%dx%dx%d" % s: t for s, t in times.items()}}
+        _store(cache, db)
+    return best, times
+
+
+def tune_uniform(mesh, cells, params, precision=0, steps=6, shapes=None, cache=DEFAULT_CACHE, refresh=False):
+    """Best brick shape for device-resident steps of a uniform level."""
+    from .hydro import HipHydro, UniformLevel
+
+    def run(shape):
+        HipHydro.set_brick_shape(*shape)
+        lev = UniformLevel(mesh, cells, params)
+        lev.run_device(1, precision=precision)
+        _, _, _, ms = lev.run_device(steps, precision=precision, time_kernel=True)
+        return ms
+
+    key = "uniform%d^3|slope%d|%s" % (mesh.n * 2, params.slope_type, params.riemann)
+    best, times = tune(key, precision, run, shapes, cache, refresh)
+    HipHydro.set_brick_shape(*best)
+    return best, times
+
+
+def tune_amr(amesh, uold, params, precision=0, steps=4, shapes=None, cache=DEFAULT_CACHE, refresh=False,
+             interpol_var=0, interpol_type=1, key=None):
+    """Best single brick shape for batched steps of every level of an AMR mesh."""
+    from .hydro import AmrHierarchy, HipHydro
+
+    def run(shape):
+        HipHydro.set_brick_shape(*shape)
+        h = AmrHierarchy(amesh, uold, params, interpol_var=interpol_var, interpol_type=interpol_type)
+        h.run_device(2, precision=precision, use_graph=True)
+        return h.run_device(steps, precision=precision, use_graph=True)[3]
+
+    key = key or "amr%d-%d|octs%d|slope%d|%s" % (amesh.levelmin, amesh.levelmax, amesh.ngrid, params.slope_type,
+                                                   params.riemann)
+    best, times = tune(key, precision, run, shapes, cache, refresh)
+    HipHydro.set_brick_shape(*best)
+    return best, times
+
+
+def fill_efficiency(noct, n_bricks, shape):
+    """Fraction of brick slots holding an oct of the level."""
+    return noct / float(n_bricks * int(np.prod(shape)))

@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -23,9 +24,38 @@ static hipStream_t g_stream = nullptr;  // library stream, shared by all context
 State &st() { return *g_cur; }
 void set_error(const std::string &msg) { g_cur->last_error = msg; }
 
+// Device blocks released by a context are kept for reuse by a later request
+// of exactly the same size (contexts of one mesh size come and go in ensemble
+// serving), up to kCacheCap bytes.  hipFree synchronises the device; a cached
+// block needs no synchronisation because every use of it is ordered on the
+// one library stream.
+static std::multimap<size_t, void *> g_cache;
+static size_t g_cache_bytes = 0;
+static constexpr size_t kCacheCap = size_t(32) << 30;
+
+static void cache_release_all()
+{
+    for (auto &kv : g_cache) (void)hipFree(kv.second);
+    g_cache.clear();
+    g_cache_bytes = 0;
+}
+
 hipError_t dmalloc(void **p, size_t n)
 {
-    hipError_t e = hipMalloc(p, n);
+    auto it = g_cache.find(n);
+    hipError_t e = hipSuccess;
+    if (it != g_cache.end()) {
+        *p = it->second;
+        g_cache.erase(it);
+        g_cache_bytes -= n;
+    } else {
+        e = hipMalloc(p, n);
+        if (e == hipErrorOutOfMemory && !g_cache.empty()) {
+            (void)hipGetLastError();
+            cache_release_all();
+            e = hipMalloc(p, n);
+        }
+    }
     if (e == hipSuccess) {
         g_cur->allocs[*p] = n;
         g_cur->dev_bytes += (int64_t)n;
@@ -39,12 +69,27 @@ void dfree(void *p)
 {
     if (!p) return;
     auto it = g_cur->allocs.find(p);
-    if (it != g_cur->allocs.end()) {
-        g_cur->dev_bytes -= (int64_t)it->second;
-        g_cur->allocs.erase(it);
+    if (it == g_cur->allocs.end()) {
+        (void)hipFree(p);
+        return;
     }
-    (void)hipFree(p);
+    const size_t n = it->second;
+    g_cur->dev_bytes -= (int64_t)n;
+    g_cur->allocs.erase(it);
+    if (g_cache_bytes + n <= kCacheCap) {
+        g_cache.emplace(n, p);
+        g_cache_bytes += n;
+    } else {
+        (void)hipFree(p);
+    }
 }
+
+int64_t cached_bytes() { return (int64_t)g_cache_bytes; }
+void release_cached() { cache_release_all(); }
+
+// Pinned host buffer of small synchronous reads (shared: one host thread
+// drives the library).
+static double *g_hred = nullptr;
 
 int ctx_count() { return (int)g_ctx.size(); }
 State *ctx_get(int id) { return id >= 0 && id < (int)g_ctx.size() ? g_ctx[id] : nullptr; }
@@ -401,7 +446,8 @@ int rh_init(void)
     if (!g_stream) RH_HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     s.stream = g_stream;
     RH_HIP_CHECK(dmalloc((void **)&s.red_out, 8 * sizeof(double)));
-    RH_HIP_CHECK(hipHostMalloc(&s.h_red, 8 * sizeof(double), hipHostMallocDefault));
+    if (!g_hred) RH_HIP_CHECK(hipHostMalloc(&g_hred, 8 * sizeof(double), hipHostMallocDefault));
+    s.h_red = g_hred;
     s.red_cap = 4096;
     RH_HIP_CHECK(dmalloc((void **)&s.red_part, (size_t)s.red_cap * 4 * sizeof(double)));
     RH_HIP_CHECK(dmalloc((void **)&s.dt_dev, 2 * sizeof(double)));
@@ -418,7 +464,8 @@ int rh_finalize(void)
 {
     State &s = st();
     if (!s.inited) return 0;
-    (void)hipStreamSynchronize(s.stream);
+    // no device synchronisation: released blocks go to the allocation cache
+    // and are reused only by work ordered after this context's on the stream
     free_level_tables();
     dfree(s.uold);
     dfree(s.unew);
@@ -434,7 +481,6 @@ int rh_finalize(void)
     dfree(s.d_counter);
     dfree(s.red_part);
     dfree(s.red_out);
-    (void)hipHostFree(s.h_red);
     dfree(s.dt_dev);
     dfree(s.t_dev);
     (void)hipEventDestroy(s.ev0);
@@ -738,17 +784,27 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
         lb.bz = s.brick_bz;
         // enumerate the bricks that hold at least one oct of the level
         std::vector<int64_t> keys;
-        keys.reserve(num_octs);
         const int64_t nbx = (c.boxmax[0] - c.boxmin[0] + lb.bx - 1) / lb.bx;
         const int64_t nby = (c.boxmax[1] - c.boxmin[1] + lb.by - 1) / lb.by;
-        for (int i = head_idx - 1; i < head_idx - 1 + num_octs; i++) {
-            const int64_t bx = (s.h_ckey[3 * (size_t)i] - c.boxmin[0]) / lb.bx;
-            const int64_t by = (s.h_ckey[3 * (size_t)i + 1] - c.boxmin[1]) / lb.by;
-            const int64_t bz = (s.h_ckey[3 * (size_t)i + 2] - c.boxmin[2]) / lb.bz;
-            keys.push_back((bz * nby + by) * nbx + bx);
+        const int64_t nbz = (c.boxmax[2] - c.boxmin[2] + lb.bz - 1) / lb.bz;
+        auto brick_of = [&](size_t i) {
+            const int64_t bx = (s.h_ckey[3 * i] - c.boxmin[0]) / lb.bx;
+            const int64_t by = (s.h_ckey[3 * i + 1] - c.boxmin[1]) / lb.by;
+            const int64_t bz = (s.h_ckey[3 * i + 2] - c.boxmin[2]) / lb.bz;
+            return (bz * nby + by) * nbx + bx;
+        };
+        if (nbx * nby * nbz <= (int64_t(1) << 26)) {
+            // occupancy map of the level's brick grid: linear in octs + bricks
+            std::vector<uint8_t> occ((size_t)(nbx * nby * nbz), 0);
+            for (int i = head_idx - 1; i < head_idx - 1 + num_octs; i++) occ[(size_t)brick_of((size_t)i)] = 1;
+            for (size_t k = 0; k < occ.size(); k++)
+                if (occ[k]) keys.push_back((int64_t)k);
+        } else {
+            keys.reserve(num_octs);
+            for (int i = head_idx - 1; i < head_idx - 1 + num_octs; i++) keys.push_back(brick_of((size_t)i));
+            std::sort(keys.begin(), keys.end());
+            keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
         }
-        std::sort(keys.begin(), keys.end());
-        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
         lb.nbricks = (int)keys.size();
         std::vector<int32_t> org(3 * keys.size());
         for (size_t b = 0; b < keys.size(); b++) {

@@ -173,6 +173,69 @@ __device__ __forceinline__ void gather_cell(const double *__restrict__ uold, con
     gather_cell_m(uold, refined, v, okey, ndim, boxmin, boxmax, reflexive_mask(btype), c, u, rf);
 }
 
+// gather_cell_m for 3D split into the address computation and the loads, so
+// that the loads of the next plane can be issued a plane ahead: the cell's
+// five values start at p (stride 8; null: no oct, zero state), its refined
+// flag is *rf (null: 0), and momentum component flip (0..2, or -1) is negated
+// when the values are used (reflexive boundary ghost).
+struct CellRef {
+    const double *p;
+    const uint8_t *rf;
+    int flip;
+};
+
+__device__ __forceinline__ CellRef cell_ref3(const double *uold, const uint8_t *refined, int v, const int okey[3],
+                                             const int boxmin[3], const int boxmax[3], int refl, const int c[3])
+{
+    CellRef r;
+    r.flip = -1;
+    if (v > 0) {
+        const int ind = c[0] + 2 * c[1] + 4 * c[2];
+        r.p = uold + (size_t)(v - 1) * 40 + ind;
+        r.rf = refined + (size_t)(v - 1) * 8 + ind;
+        return r;
+    }
+    if (v == 0) {
+        r.p = nullptr;
+        r.rf = nullptr;
+        return r;
+    }
+    int dir = 0, side = 0;
+    for (int d = 2; d >= 0; d--) {
+        if (okey[d] < boxmin[d]) { dir = d; side = 0; }
+        else if (okey[d] >= boxmax[d]) { dir = d; side = 1; }
+    }
+    const int type = (refl >> (2 * dir + side)) & 1 ? 1 : 2;
+    int s3[3] = {c[0], c[1], c[2]}, rs[3] = {c[0], c[1], c[2]};
+    const bool rvalid = side == 0 ? c[dir] == 1 : c[dir] == 0;
+    rs[dir] = side == 0 ? 0 : 1;
+    s3[dir] = type == 1 ? 1 - c[dir] : (side == 0 ? 0 : 1);
+    const int ref = -v;
+    r.p = uold + (size_t)(ref - 1) * 40 + (s3[0] + 2 * s3[1] + 4 * s3[2]);
+    r.rf = rvalid ? refined + (size_t)(ref - 1) * 8 + rs[0] + 2 * rs[1] + 4 * rs[2] : nullptr;
+    r.flip = type == 1 ? dir : -1;
+    return r;
+}
+
+__device__ __forceinline__ void cell_load3(const CellRef &r, double u[5], uint8_t &rf)
+{
+    if (r.p) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) u[n] = r.p[8 * n];
+    } else {
+#pragma unroll
+        for (int n = 0; n < 5; n++) u[n] = 0.0;
+    }
+    rf = r.rf ? *r.rf : (uint8_t)0;
+}
+
+__device__ __forceinline__ void cell_flip(double u[5], int flip)
+{
+    u[1] = flip == 0 ? -u[1] : u[1];
+    u[2] = flip == 1 ? -u[2] : u[2];
+    u[3] = flip == 2 ? -u[3] : u[3];
+}
+
 template <typename T>
 __device__ __forceinline__ HydroK<T> make_k(const double dt, double dx, double gamma, double smallr, double smallc2,
                                             double courant_factor, double smax_cap, int use_smax, int slope,
@@ -250,6 +313,14 @@ __device__ __forceinline__ void reflux_add(double *unew, const int32_t *ghost_pa
     for (int n = 0; n < 5; n++) atomicAdd(p + n * ttd, sign * (F[n] * w));
 }
 
+// Loads of the next plane's P-phase cells issued a plane ahead, per precision.
+#ifndef RH_PREFETCH_F64
+#define RH_PREFETCH_F64 1
+#endif
+#ifndef RH_PREFETCH_F32
+#define RH_PREFETCH_F32 0
+#endif
+
 // LDS of one brick workgroup.  NS primitive planes are kept: 2 when the slopes
 // need only face neighbours (the z neighbours of a column sit in registers),
 // 3 for the 27-point slope of slope_type 3.
@@ -316,27 +387,97 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     T qcol_m[5] = {T(0), T(0), T(0), T(0), T(0)}, qcol_0[5] = {T(0), T(0), T(0), T(0), T(0)};
 
 
+    // P-phase cells of this thread: its own column's cell, and for the first
+    // NRING threads one cell of the ring around the column region.  Their
+    // loads are issued one plane ahead (u_o, u_r) and converted the next plane.
+    constexpr int NRING = NP - NT;
+    int ring_i = 0;
+    if (tid < NRING) {
+        const int r = tid;
+        int rx, ry;
+        if (r < PX) {
+            rx = r - 2;
+            ry = -2;
+        } else if (r < 2 * PX) {
+            rx = r - PX - 2;
+            ry = 2 * BY + 1;
+        } else if (r < 2 * PX + 2 * BY + 2) {
+            rx = -2;
+            ry = r - 2 * PX - 1;
+        } else {
+            rx = 2 * BX + 1;
+            ry = r - 2 * PX - (2 * BY + 2) - 1;
+        }
+        ring_i = (ry + 2) * PX + (rx + 2);
+    }
+    auto fetch = [&](int i, int tz, double u[5], uint8_t &rf, int &fl) {
+        const int px = i % PX - 2, py = i / PX - 2;
+        const int ox = (px + 2) >> 1, oy = (py + 2) >> 1, oz = (tz + 2) >> 1;
+        const int v = swin[(oz * WY + oy) * WX + ox];
+        const int okey[3] = {ox0 + ox - 1, oy0 + oy - 1, oz0 + oz - 1};
+        const int c[3] = {px & 1, py & 1, tz & 1};
+        const CellRef r = cell_ref3(a.uold, a.refined, v, okey, boxmin, boxmax, refl, c);
+        cell_load3(r, u, rf);
+        fl = r.flip;
+    };
+    // prefetch a plane ahead (about 45 more VGPRs, load latency hidden): only
+    // where the registers are free without losing a wave per SIMD, i.e. the
+    // FP64 HLLC variant (183 -> 230 of 256); elsewhere the loads are issued at
+    // use, as the other variants would spill or drop to fewer waves
+    constexpr bool PF = RH_PREFETCH_F64 && sizeof(T) == 8 && RS == 3 && NS == 2;
+    double u_o[5], u_r[5];
+    uint8_t rf_o, rf_r = 0;
+    int fl_o, fl_r = -1;
+    if (PF) {
+        fetch(pc, -2, u_o, rf_o, fl_o);
+        if (tid < NRING) fetch(ring_i, -2, u_r, rf_r, fl_r);
+    }
+
     for (int t = -2; t <= 2 * BZ + 1; ++t) {
         const int st = NS == 3 ? ring_slot3(t) : ring_slot2(t);
         // ---- P: primitives of plane t ----
-        for (int i = tid; i < NP; i += NT) {
-            const int px = i % PX - 2, py = i / PX - 2;
-            const int ox = (px + 2) >> 1, oy = (py + 2) >> 1, oz = (t + 2) >> 1;
-            const int v = swin[(oz * WY + oy) * WX + ox];
-            int okey[3] = {ox0 + ox - 1, oy0 + oy - 1, oz0 + oz - 1};
-            int c[3] = {px & 1, py & 1, t & 1};
-            double u[5], q[5];
-            uint8_t rf;
-            gather_cell_m(a.uold, a.refined, v, okey, 3, boxmin, boxmax, refl, c, u, rf);
-            prim_from_cons(u, q, a.smallr, smalle, a.gamma, gdt);
-#pragma unroll
-            for (int n = 0; n < 5; n++) sq[st][n][i] = T(q[n]);
-            sref[st][i] = rf;
-        }
-        __syncthreads();
         T qcol_p[5];
+        if constexpr (!PF) {
+            for (int i = tid; i < NP; i += NT) {
+                const int px = i % PX - 2, py = i / PX - 2;
+                const int ox = (px + 2) >> 1, oy = (py + 2) >> 1, oz = (t + 2) >> 1;
+                const int v = swin[(oz * WY + oy) * WX + ox];
+                int okey[3] = {ox0 + ox - 1, oy0 + oy - 1, oz0 + oz - 1};
+                int c[3] = {px & 1, py & 1, t & 1};
+                double u[5], q[5];
+                uint8_t rf;
+                gather_cell_m(a.uold, a.refined, v, okey, 3, boxmin, boxmax, refl, c, u, rf);
+                prim_from_cons(u, q, a.smallr, smalle, a.gamma, gdt);
 #pragma unroll
-        for (int n = 0; n < 5; n++) qcol_p[n] = sq[st][n][pc];
+                for (int n = 0; n < 5; n++) sq[st][n][i] = T(q[n]);
+                sref[st][i] = rf;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int n = 0; n < 5; n++) qcol_p[n] = sq[st][n][pc];
+        } else {
+            double q[5];
+            cell_flip(u_o, fl_o);
+            prim_from_cons(u_o, q, a.smallr, smalle, a.gamma, gdt);
+#pragma unroll
+            for (int n = 0; n < 5; n++) {
+                qcol_p[n] = T(q[n]);
+                sq[st][n][pc] = qcol_p[n];
+            }
+            sref[st][pc] = rf_o;
+            if (tid < NRING) {
+                cell_flip(u_r, fl_r);
+                prim_from_cons(u_r, q, a.smallr, smalle, a.gamma, gdt);
+#pragma unroll
+                for (int n = 0; n < 5; n++) sq[st][n][ring_i] = T(q[n]);
+                sref[st][ring_i] = rf_r;
+            }
+            if (t < 2 * BZ + 1) {
+                fetch(pc, t + 1, u_o, rf_o, fl_o);
+                if (tid < NRING) fetch(ring_i, t + 1, u_r, rf_r, fl_r);
+            }
+            __syncthreads();
+        }
         if (t < 0) {
 #pragma unroll
             for (int n = 0; n < 5; n++) {
@@ -983,6 +1124,9 @@ hipError_t launch_upload(double *uold, const int32_t *father, int head, int noct
 // ---------------------------------------------------------------------------
 #define RH_BRICK_SHAPES(X) \
     X(7, 7, 8)             \
+    X(7, 7, 16)            \
+    X(15, 7, 8)            \
+    X(15, 7, 16)           \
     X(7, 7, 4)             \
     X(3, 3, 4)             \
     X(3, 3, 8)             \

@@ -161,6 +161,87 @@ int rh_ctx_destroy(int id)
     return 0;
 }
 
+int rh_ctx_clone_mesh(int src_id)
+{
+    State *src = ctx_get(src_id);
+    State &s = st();
+    if (!src || src == &s) return efail(RH_ERR_ARG, "rh_ctx_clone_mesh: bad source context");
+    if (!src->inited || !src->uold) return efail(RH_ERR_STATE, "the source context holds no mesh");
+    if (s.uold) return efail(RH_ERR_STATE, "the current context already holds a mesh");
+    if (int r = rh_init()) return r;
+    // scalar and host-side mesh description
+    s.ndim = src->ndim;
+    s.nvar = src->nvar;
+    s.ttd = src->ttd;
+    s.ngridmax = src->ngridmax;
+    s.ncachemax = src->ncachemax;
+    s.ngridtot = src->ngridtot;
+    s.hash_cap = src->hash_cap;
+    s.nlevelmax = src->nlevelmax;
+    s.ckey_max = src->ckey_max;
+    s.key_off = src->key_off;
+    s.box_min = src->box_min;
+    s.box_max = src->box_max;
+    for (int d = 0; d < 3; d++) {
+        s.periodic[d] = src->periodic[d];
+        s.btype[d][0] = src->btype[d][0];
+        s.btype[d][1] = src->btype[d][1];
+    }
+    s.h_ckey = src->h_ckey;
+    s.h_lev = src->h_lev;
+    s.brick_bx = src->brick_bx;
+    s.brick_by = src->brick_by;
+    s.brick_bz = src->brick_bz;
+    s.interpol_var = src->interpol_var;
+    s.interpol_type = src->interpol_type;
+    s.nghost_total = src->nghost_total;
+    // device arrays: fresh state, copies of the mesh tables (all on the stream)
+    const size_t nd = (size_t)s.ngridtot * s.nvar * s.ttd * sizeof(double);
+    const size_t ng = (size_t)(s.ncachemax > 0 ? s.ncachemax : 1);
+    auto dup = [&](void **dst, const void *from, size_t bytes) -> hipError_t {
+        hipError_t e = dmalloc(dst, bytes);
+        if (e != hipSuccess) return e;
+        return hipMemcpyAsync(*dst, from, bytes, hipMemcpyDeviceToDevice, s.stream);
+    };
+    RH_HIP_CHECK(dmalloc((void **)&s.uold, nd));
+    RH_HIP_CHECK(dmalloc((void **)&s.unew, nd));
+    RH_HIP_CHECK(hipMemsetAsync(s.uold, 0, nd, s.stream));
+    RH_HIP_CHECK(hipMemsetAsync(s.unew, 0, nd, s.stream));
+    RH_HIP_CHECK(dup((void **)&s.ckey, src->ckey, (size_t)s.ngridtot * 3 * sizeof(int32_t)));
+    RH_HIP_CHECK(dup((void **)&s.lev, src->lev, (size_t)s.ngridtot * sizeof(int32_t)));
+    RH_HIP_CHECK(dup((void **)&s.refined, src->refined, (size_t)s.ngridtot * 8));
+    RH_HIP_CHECK(dup((void **)&s.hkeys, src->hkeys, (size_t)s.hash_cap * sizeof(uint64_t)));
+    RH_HIP_CHECK(dup((void **)&s.hvals, src->hvals, (size_t)s.hash_cap * sizeof(int32_t)));
+    RH_HIP_CHECK(dup((void **)&s.ghost_parent, src->ghost_parent, ng * 2 * sizeof(int32_t)));
+    RH_HIP_CHECK(dup((void **)&s.ghost_src, src->ghost_src, ng * 14 * sizeof(int32_t)));
+    RH_HIP_CHECK(dup((void **)&s.ghost_srckey, src->ghost_srckey, ng * 21 * sizeof(int32_t)));
+    RH_HIP_CHECK(dup((void **)&s.ghost_key, src->ghost_key, ng * 3 * sizeof(int32_t)));
+    const int nslot = s.ndim == 3 ? 27 : 3;
+    for (const auto &kv : src->nbors) {
+        LevelNbor ln = kv.second;
+        RH_HIP_CHECK(dup((void **)&ln.nbor, kv.second.nbor, (size_t)ln.noct * nslot * sizeof(int32_t)));
+        if (kv.second.father)
+            RH_HIP_CHECK(dup((void **)&ln.father, kv.second.father, (size_t)ln.noct * 2 * sizeof(int32_t)));
+        s.nbors[kv.first] = ln;
+    }
+    for (const auto &kv : src->bricks) {
+        LevelBricks lb = kv.second;
+        const size_t nw = (size_t)(lb.bx + 2) * (lb.by + 2) * (lb.bz + 2);
+        RH_HIP_CHECK(dup((void **)&lb.win, kv.second.win, (size_t)lb.nbricks * nw * sizeof(int32_t)));
+        RH_HIP_CHECK(dup((void **)&lb.org, kv.second.org, (size_t)lb.nbricks * 3 * sizeof(int32_t)));
+        s.bricks[kv.first] = lb;
+    }
+    return 0;
+}
+
+int rh_cache_release(int64_t *bytes_released)
+{
+    if (bytes_released) *bytes_released = cached_bytes();
+    if (ctx_get(0)->inited) RH_HIP_CHECK(hipStreamSynchronize(ctx_get(0)->stream));
+    release_cached();
+    return 0;
+}
+
 int rh_ctx_footprint(int id, int64_t *bytes)
 {
     State *s = ctx_get(id);

@@ -151,6 +151,10 @@ State &st();
 hipError_t dmalloc(void **p, size_t n);
 void dfree(void *p);
 
+// Bytes held in the allocation cache; release them to the device.
+int64_t cached_bytes();
+void release_cached();
+
 // Context table: id 0 is the default context, always present.
 int ctx_count();
 State *ctx_get(int id);

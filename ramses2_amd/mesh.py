@@ -191,15 +191,42 @@ class AmrMesh:
         self.ckey = np.concatenate(ck).astype(np.int32)
         self.lev = np.concatenate(lv).astype(np.int32)
         self.ngrid = pos
+        # per level: encoded keys in sorted order and the matching oct indices
+        self._sorted = {}
+        for l in self.levels:
+            i0 = self.head[l] - 1
+            enc = self._encode(self.ckey[i0:i0 + self.noct[l]].astype(np.int64))
+            order = np.argsort(enc, kind="stable")
+            self._sorted[l] = (enc[order], order + i0)
         self.refined = np.zeros((pos, 8), dtype=np.uint8)
-        for l in self.levels[:-1]:
-            fine = set(map(tuple, self.ckey[self.head[l + 1] - 1:self.head[l + 1] - 1 + self.noct[l + 1]]))
-            for i in range(self.head[l] - 1, self.head[l] - 1 + self.noct[l]):
-                c = self.ckey[i]
-                for ind in range(8):
-                    cc = (2 * c[0] + (ind & 1), 2 * c[1] + ((ind >> 1) & 1), 2 * c[2] + ((ind >> 2) & 1))
-                    if cc in fine:
-                        self.refined[i, ind] = 1
+        for l in self.levels[1:]:
+            if l - 1 not in self.head:
+                continue
+            i0 = self.head[l] - 1
+            par, ind = self.parent_cells(l, self.ckey[i0:i0 + self.noct[l]].astype(np.int64))
+            self.refined[par, ind] = 1
+
+    @staticmethod
+    def _encode(k):
+        return k[:, 0] | (k[:, 1] << 21) | (k[:, 2] << 42)
+
+    def find(self, l, keys):
+        """0-based indices of the octs of level l with the given keys (-1 if absent)."""
+        enc, idx = self._sorted[l]
+        q = self._encode(np.asarray(keys, dtype=np.int64).reshape(-1, 3))
+        pos = np.searchsorted(enc, q)
+        pos = np.minimum(pos, len(enc) - 1)
+        hit = enc[pos] == q
+        return np.where(hit, idx[pos], -1)
+
+    def parent_cells(self, l, keys):
+        """(parent oct index 0-based, cell 0..7) at level l-1 of level-l octs with these keys."""
+        k = np.asarray(keys, dtype=np.int64).reshape(-1, 3)
+        par = self.find(l - 1, k >> 1)
+        if np.any(par < 0):
+            raise ValueError("level %d octs without a parent cell at level %d" % (l, l - 1))
+        ind = (k[:, 0] & 1) + 2 * (k[:, 1] & 1) + 4 * (k[:, 2] & 1)
+        return par, ind
 
     def dx(self, l):
         return self.boxlen / 2 ** l

@@ -126,6 +126,7 @@ class HipEngine:
         self.p = params or HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
         self.precision = precision
         self._mesh = {}
+        self._templates = {}
         call("rh_init")
         if brick is None:
             v = [ctypes.c_int32() for _ in range(3)]
@@ -151,8 +152,10 @@ class HipEngine:
         call("rh_estimate_uniform_bytes", mesh.n, 0, 2 * mesh.noct, ctypes.byref(b))
         return b.value
 
-    def open(self, spec: JobSpec):
-        mesh, grid, (ckey_max, key_off, box_min, box_max) = self._mesh_for(spec.ncell)
+    def _build(self, ncell):
+        """A context holding the mesh of an ncell^3 periodic box: level
+        parameters, oct records, device hash, neighbour and brick tables."""
+        mesh, grid, (ckey_max, key_off, box_min, box_max) = self._mesh_for(ncell)
         cid = ctypes.c_int32()
         call("rh_ctx_create", ctypes.byref(cid))
         cid = cid.value
@@ -171,6 +174,39 @@ class HipEngine:
             bmax = np.ascontiguousarray(box_max[l])
             call("rh_build_nbor", 1, mesh.noct, 2 * mesh.noct, int(ckey_max[l]), int(key_off[l]),
                  ptr(bmin, _lib._ip), ptr(bmax, _lib._ip), ptr(per, _lib._ip))
+        except Exception:
+            call("rh_ctx_destroy", cid)
+            raise
+        finally:
+            call("rh_ctx_select", 0)
+        return cid
+
+    def _template(self, ncell):
+        t = self._templates.get(ncell)
+        if t is None:
+            t = self._build(ncell)
+            self._templates[ncell] = t
+        return t
+
+    def warm(self, ncell):
+        """Build the template context of a grid size ahead of its first job."""
+        self._template(ncell)
+
+    def template_bytes(self):
+        return sum(self.held_bytes(t) for t in self._templates.values())
+
+    def open(self, spec: JobSpec):
+        """A new job context: the mesh cloned from the size's template (device
+        copies, no host work proportional to the mesh), the initial condition
+        written on the device, the first dt reduced on the device."""
+        mesh = self._mesh_for(spec.ncell)[0]
+        tid = self._template(spec.ncell)
+        cid = ctypes.c_int32()
+        call("rh_ctx_create", ctypes.byref(cid))
+        cid = cid.value
+        call("rh_ctx_select", cid)
+        try:
+            call("rh_ctx_clone_mesh", tid)
             a = np.ascontiguousarray(spec.ic, dtype=np.float64)
             call("rh_init_ic", 1, mesh.noct, spec.ncell, spec.kind, ptr(a), self.p.gamma)
             g = np.zeros(3)
@@ -182,6 +218,12 @@ class HipEngine:
         finally:
             call("rh_ctx_select", 0)
         return cid
+
+    def close_all(self):
+        """Destroy the template contexts."""
+        for t in self._templates.values():
+            call("rh_ctx_destroy", t)
+        self._templates.clear()
 
     def step(self, handles, nsteps=1):
         ids = np.ascontiguousarray(handles, dtype=np.int32)

@@ -59,12 +59,16 @@ def test_godunov3d_floors_fp64():
     assert np.all(err < 1e-10), err
 
 
-@pytest.mark.parametrize("brick", [(7, 7, 8), (3, 3, 4), (1, 1, 1), (3, 3, 8), (7, 7, 4)])
-def test_brick_shapes_agree(brick):
-    """Every compiled brick shape gives the oracle's answer."""
+@pytest.mark.parametrize("variant", [(2, "hllc"), (3, "llf"), (1, "hll")])
+@pytest.mark.parametrize("brick", [(7, 7, 8), (7, 7, 16), (15, 7, 8), (15, 7, 16), (3, 3, 4), (1, 1, 1), (3, 3, 8),
+                                   (7, 7, 4)])
+def test_brick_shapes_agree(brick, variant):
+    """Every compiled brick shape, in each kernel variant (HLLC; the 27-point
+    slope's three-plane ring; the generic solver path), gives the oracle's
+    answer."""
     rng = np.random.default_rng(11)
     cells = ic.random_state((16, 16, 16), rng)
-    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=variant[0], riemann=variant[1])
     try:
         g, o, _, _, _ = _one_step_3d(4, cells, p, brick=brick)
     finally:
