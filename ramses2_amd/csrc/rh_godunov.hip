@@ -424,7 +424,9 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     // where the registers are free without losing a wave per SIMD, i.e. the
     // FP64 HLLC variant (183 -> 230 of 256); elsewhere the loads are issued at
     // use, as the other variants would spill or drop to fewer waves
-    constexpr bool PF = RH_PREFETCH_F64 && sizeof(T) == 8 && RS == 3 && NS == 2;
+    // (it also needs a thread for every ring cell: not the 1x1x1 brick, 16
+    // threads for 20 ring cells)
+    constexpr bool PF = (sizeof(T) == 8 ? RH_PREFETCH_F64 : RH_PREFETCH_F32) && RS == 3 && NS == 2 && NP - NT <= NT;
     double u_o[5], u_r[5];
     uint8_t rf_o, rf_r = 0;
     int fl_o, fl_r = -1;

@@ -228,3 +228,22 @@ def test_sod_matches_exact_solution():
     assert l1 < 2e-3, l1
     # mass is conserved up to the boundary fluxes (zero-gradient ends carry none at rest)
     assert abs(cells[0].sum() - ic.sod_1d(1024)[0].sum()) < 1e-12 * 1024
+
+
+def test_sub_box_equals_full_level():
+    """A periodic computational box of 8 octs per dimension inside the 16-oct
+    level (box_ckey_max = 8) steps exactly like the full 8-oct level with the
+    same dx: the box bounds alone decide the periodic wrap."""
+    periodic = True
+    rng = np.random.default_rng(21)
+    cells = ic.random_state((16, 16, 16), rng)
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
+    full = UniformMesh(3, 4)                       # 8 octs per dimension, dx = 1/16
+    sub = UniformMesh(3, 5, boxlen=2.0, n=8)       # 8 of 16 octs, dx = 2/32
+    assert full.dx == sub.dx and sub.box() == (5, [0, 0, 0], [8, 8, 8])
+    out = []
+    for m in (full, sub):
+        om = oracle_uniform(m, m.from_cells(cells), periodic=periodic)
+        om.godunov_fine(m.level, 1e-3, oracle_params(p))
+        out.append(m.to_cells(om.unew))
+    assert np.array_equal(out[0], out[1])
