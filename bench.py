@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--amr-steps", type=int, default=20)
     ap.add_argument("--amr-levels", type=str, default="6,10", help="levelmin,levelmax of the AMR Sedov case")
     ap.add_argument("--no-amr", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the 7x7x8 brick shape (no autotuning)")
+    ap.add_argument("--fp64-brick", type=str, default=None, help="brick shape bx,by,bz of the FP64 run (no tuning)")
+    ap.add_argument("--mixed-brick", type=str, default=None, help="brick shape of the mixed run (no tuning)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -115,8 +118,11 @@ def load_traffic():
     return d.get("hbm_bytes_per_launch")
 
 
-def run_gpu(args, p, cells, mesh, precision):
+def run_gpu(args, p, cells, mesh, precision, brick=None):
     from ramses2_amd import UniformLevel, _lib
+    from ramses2_amd.hydro import HipHydro
+    if brick is not None:
+        HipHydro.set_brick_shape(*brick)
     lev = UniformLevel(mesh, cells, p)
     if args.warmup:
         lev.run_device(args.warmup, precision=precision)
@@ -147,19 +153,21 @@ def run_amr(args):
     cells = int(sum(am.noct[l] for l in am.levels) * 8)
     print("bench: AMR Sedov levels %d-%d, octs per level %s, %d cells (mesh built in %.1f s)"
           % (lmin, lmax, {l: am.noct[l] for l in am.levels}, cells, t_build), file=sys.stderr, flush=True)
-    best, times = tune_amr(am, u0, p, precision=0, cache=None, key="sedov-amr")
-    print("bench: AMR brick shapes (ms/step): %s -> %s" % ({"%dx%dx%d" % k: round(v, 3) for k, v in times.items()},
-                                                           best), file=sys.stderr, flush=True)
+    if args.no_tune:
+        bricks, times = {l: (7, 7, 8) for l in am.levels}, {}
+    else:
+        bricks, times = tune_amr(am, u0, p, precision=0, cache=None, key="sedov-amr")
+    print("bench: AMR brick shapes (ms/step): %s -> %s" % ({k: round(v, 3) for k, v in times.items()}, bricks),
+          file=sys.stderr, flush=True)
     out = {"workload": "3D Sedov AMR, levels %d-%d, refinement on relative density/pressure gradient > 0.1 "
                        "(hydro_refine criterion on the analytic blast of radius 0.3), gamma 1.4, MonCen, HLLC, "
                        "Courant 0.5, periodic, nsubcycle=1" % (lmin, lmax),
            "octs_per_level": {str(l): int(am.noct[l]) for l in am.levels}, "cells": cells,
-           "brick_shape": "%dx%dx%d" % best,
-           "brick_shape_ms_per_step": {"%dx%dx%d" % k: v for k, v in times.items()}}
+           "brick_shapes": {str(l): "%dx%dx%d" % b for l, b in bricks.items()},
+           "brick_shape_ms_per_step": times or None}
     states = {}
     for prec, name in ((0, "fp64"), (1, "mixed")):
-        HipHydro.set_brick_shape(*best)
-        h = AmrHierarchy(am, u0, p)
+        h = AmrHierarchy(am, u0, p, bricks=bricks)
         h.run_device(2, precision=prec, use_graph=True)
         dt, tt, sums, ms = h.run_device(args.amr_steps, precision=prec, use_graph=True)
         states[name] = (h.state(), sums)
@@ -177,8 +185,7 @@ def run_amr(args):
     out["mixed"]["l1_rel_vs_fp64_leaf"] = l1
     out["mixed"]["mass_drift_vs_fp64"] = float((sm[0] - s64[0]) / s64[0])
     # the drop-in sequence: per level courant (synchronous), set_unew, godunov, set_uold, upload
-    HipHydro.set_brick_shape(*best)
-    h = AmrHierarchy(am, u0, p)
+    h = AmrHierarchy(am, u0, p, bricks=bricks)
     h.step()
     from ramses2_amd import _lib
     _lib.call("rh_device_sync")
@@ -210,7 +217,28 @@ def run_serving(args):
     # capacity: serve_slots jobs of the stream resident, 20 batched steps timed
     # (admission timed apart: it is host work between ticks)
     cal = base[:args.serve_slots]
-    eng = HipEngine(precision=0)
+    # brick shape for the serving mix: batched FP64 ticks of the calibration jobs per shape
+    shape_ms = {}
+    if not args.no_tune:
+        from ramses2_amd.autotune import compiled_shapes
+        for sh in compiled_shapes():
+            e = HipEngine(precision=0, brick=sh)
+            hs = [e.open(s) for s in cal]
+            e.step(hs, 2)
+            e.sync()
+            t0 = time.perf_counter()
+            e.step(hs, 10)
+            e.sync()
+            shape_ms[sh] = (time.perf_counter() - t0) / 10 * 1e3
+            for h in hs:
+                e.close(h)
+            e.close_all()
+        brick = min(shape_ms, key=shape_ms.get)
+        print("bench: serving brick shapes (ms per tick of %d jobs): %s -> %s" % (
+            len(cal), {"%dx%dx%d" % k: round(v, 3) for k, v in shape_ms.items()}, brick), file=sys.stderr, flush=True)
+    else:
+        brick = (7, 7, 8)
+    eng = HipEngine(precision=0, brick=brick)
     # one template context per grid size of the stream (the server's start-up)
     t0 = time.perf_counter()
     for n in sorted({s.ncell for s in base}):
@@ -246,7 +274,8 @@ def run_serving(args):
            "FP64 steps timed" % (len(cal), ncal), "admission_ms_per_job": t_open * 1e3,
            "release_ms_per_job": t_close * 1e3, "template_build_s": t_warm,
            "template_bytes": eng.template_bytes(), "mean_job_cells": mean_cells, "steps_per_job": args.serve_steps,
-           "max_slots": args.serve_slots, "hbm_budget_bytes": budget,
+           "max_slots": args.serve_slots, "hbm_budget_bytes": budget, "brick_shape": "%dx%dx%d" % brick,
+           "brick_shape_ms_per_tick": {"%dx%dx%d" % k: v for k, v in shape_ms.items()} or None,
            "sizes": "N^3, N log-uniform in [32,128], even; 50% blast, 50% Riemann; gamma 1.4, MonCen, HLLC, C 0.5"}
     for prec, name in ((0, "fp64"), (1, "mixed")):
         eng.precision = prec
@@ -293,7 +322,24 @@ def main():
         print("bench: copy probe %d B read + %d B written in %.3f ms (%.0f GB/s)"
               % (nbytes, nbytes, ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9), file=sys.stderr, flush=True)
 
-    lev64, el64, kms64, dt64, t64, sums64 = run_gpu(args, p, cells, mesh, precision=0)
+    bricks = {0: (7, 7, 8), 1: (7, 7, 8)}
+    fixed = {}
+    for prec, opt in ((0, args.fp64_brick), (1, args.mixed_brick)):
+        if opt:
+            fixed[prec] = bricks[prec] = tuple(int(x) for x in opt.split(","))
+    tuning = {}
+    if not args.no_tune:
+        from ramses2_amd.autotune import tune_uniform
+        for prec in [q for q in ((0, 1) if not args.no_mixed else (0,)) if q not in fixed]:
+            best, times = tune_uniform(mesh, cells, p, precision=prec, steps=6, cache=None)
+            bricks[prec] = best
+            tuning["fp64" if prec == 0 else "mixed"] = {"best": "%dx%dx%d" % best, "ms_per_step": {
+                "%dx%dx%d" % k: v for k, v in times.items()}}
+            print("bench: brick shapes %s (Godunov ms): %s -> %s" % (
+                "fp64" if prec == 0 else "mixed", {"%dx%dx%d" % k: round(v, 3) for k, v in times.items()}, best),
+                file=sys.stderr, flush=True)
+
+    lev64, el64, kms64, dt64, t64, sums64 = run_gpu(args, p, cells, mesh, precision=0, brick=bricks[0])
     u64 = lev64.state()
     value = ncells * args.steps / el64
     ms_per_step = el64 / args.steps * 1e3
@@ -302,7 +348,7 @@ def main():
 
     mixed = None
     if not args.no_mixed:
-        levm, elm, kmsm, dtm, tm, sumsm = run_gpu(args, p, cells, mesh, precision=1)
+        levm, elm, kmsm, dtm, tm, sumsm = run_gpu(args, p, cells, mesh, precision=1, brick=bricks[1])
         um = levm.state()
         l1 = {}
         names = ["rho", "mom_x", "mom_y", "mom_z", "E"]
@@ -322,7 +368,7 @@ def main():
         traffic = load_traffic()
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "kernel": "k_godunov3d<7,7,8,double> (FP64)", "kernel_ms": kms64,
+                "kernel": "k_godunov3d<%d,%d,%d,double> (FP64)" % bricks[0], "kernel_ms": kms64,
                 "bytes_per_launch": BYTES_PER_CELL * ncells}
 
     amr = None
@@ -354,7 +400,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "3D supersonic decaying turbulence, uniform %d^3, gamma=5/3, MonCen, HLLC, "
                                "Courant 0.5, periodic; full level step (courant_fine + godunov_fine)" % n,
-                   "cells": ncells, "octs": mesh.noct, "parallelism": "single GPU"},
+                   "cells": ncells, "octs": mesh.noct, "parallelism": "single GPU",
+                   "brick_shape": {"fp64": "%dx%dx%d" % bricks[0], "mixed": "%dx%dx%d" % bricks[1]},
+                   "autotune": tuning or None},
         "mixed": mixed,
         "roofline": roof,
         "cpu_baseline": cpu,

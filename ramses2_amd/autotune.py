@@ -91,20 +91,62 @@ def tune_uniform(mesh, cells, params, precision=0, steps=6, shapes=None, cache=D
 
 def tune_amr(amesh, uold, params, precision=0, steps=4, shapes=None, cache=DEFAULT_CACHE, refresh=False,
              interpol_var=0, interpol_type=1, key=None):
-    """Best single brick shape for batched steps of every level of an AMR mesh."""
+    """Brick shapes for batched steps of an AMR mesh: every compiled shape on
+    all levels, and the per-level assignment of level_shapes, each timed on
+    the mesh; returns ({level: shape}, {candidate: ms per step})."""
     from .hydro import AmrHierarchy, HipHydro
 
-    def run(shape):
-        HipHydro.set_brick_shape(*shape)
-        h = AmrHierarchy(amesh, uold, params, interpol_var=interpol_var, interpol_type=interpol_type)
+    per_level = level_shapes(amesh, shapes)
+
+    def run(cand):
+        bricks = per_level if cand == "per-level" else {l: cand for l in amesh.levels}
+        h = AmrHierarchy(amesh, uold, params, interpol_var=interpol_var, interpol_type=interpol_type, bricks=bricks)
         h.run_device(2, precision=precision, use_graph=True)
         return h.run_device(steps, precision=precision, use_graph=True)[3]
 
     key = key or "amr%d-%d|octs%d|slope%d|%s" % (amesh.levelmin, amesh.levelmax, amesh.ngrid, params.slope_type,
                                                    params.riemann)
-    best, times = tune(key, precision, run, shapes, cache, refresh)
-    HipHydro.set_brick_shape(*best)
-    return best, times
+    cands = list(shapes or compiled_shapes()) + ["per-level"]
+    ck = "%s|prec%d" % (key, precision)
+    db = _load(cache) if cache else {}
+    if not refresh and ck in db:
+        times = db[ck]["ms"]
+    else:
+        times = {("%dx%dx%d" % c if c != "per-level" else c): float(run(c)) for c in cands}
+        if cache:
+            db = _load(cache)
+            db[ck] = {"ms": times}
+            _store(cache, db)
+    best = min(times, key=times.get)
+    bricks = per_level if best == "per-level" else {l: tuple(int(x) for x in best.split("x")) for l in amesh.levels}
+    HipHydro.set_brick_shape(*bricks[amesh.levelmin])
+    return bricks, times
+
+
+def brick_work(shape):
+    """Relative work of one brick: the P phase converts (2BX+4)(2BY+4) cells
+    per plane over 2BZ+4 planes, the T/F/U phases run (2BX+2)(2BY+2) columns
+    over 2BZ+2 planes; a P cell costs about a third of a T/F/U column-plane."""
+    bx, by, bz = shape
+    return 0.35 * (2 * bx + 4) * (2 * by + 4) * (2 * bz + 4) + (2 * bx + 2) * (2 * by + 2) * (2 * bz + 2)
+
+
+def count_bricks(keys, shape, box_min=(0, 0, 0)):
+    """Bricks holding at least one of the octs (oct keys, (n,3)), on the grid
+    rh_build_nbor uses (origins at box_min + multiples of the shape)."""
+    k = (np.asarray(keys, dtype=np.int64) - np.asarray(box_min)) // np.asarray(shape)
+    return len(np.unique(k[:, 0] | (k[:, 1] << 21) | (k[:, 2] << 42)))
+
+
+def level_shapes(amesh, shapes=None):
+    """Per level, the shape with the least modelled work (bricks x brick_work)."""
+    shapes = shapes or [s for s in SHAPES if s != (1, 1, 1)]
+    out = {}
+    for l in amesh.levels:
+        i0 = amesh.head[l] - 1
+        k = amesh.ckey[i0:i0 + amesh.noct[l]]
+        out[l] = min(shapes, key=lambda s: count_bricks(k, s) * brick_work(s))
+    return out
 
 
 def fill_efficiency(noct, n_bricks, shape):

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <string>
 #include <vector>
 
@@ -70,12 +71,14 @@ hipError_t enqueue_amr_step(State &s, int parity, int precision)
     for (int l : m.levels)
         if (l > m.levelmin && (e = refresh_level_ghosts(s, l, ucur)) != hipSuccess) return e;
     if ((e = launch_zero(m.racc, 5l * std::max(s.nghost_total, 1), s.stream)) != hipSuccess) return e;
-    WorkList w;
-    w.descs = m.d_descs + parity * m.ndesc;
-    w.items = m.d_items;
-    for (int r = 0; r <= 8; r++) w.range[r] = m.range[r];
-    if ((e = launch_godunov3d_wl(w, m.bx, m.by, m.bz, m.slope, m.riemann, precision, s.stream)) != hipSuccess)
-        return e;
+    for (const auto &gr : m.groups) {
+        WorkList w;
+        w.descs = m.d_descs + parity * m.ndesc;
+        w.items = m.d_items + gr.first;
+        for (int r = 0; r <= 8; r++) w.range[r] = (int)(((long)gr.count * r) / 8);
+        if ((e = launch_godunov3d_wl(w, gr.bx, gr.by, gr.bz, m.slope, m.riemann, precision, s.stream)) != hipSuccess)
+            return e;
+    }
     if ((e = launch_reflux_fold(unext, m.racc, s.ghost_parent, s.nghost_total, 8, s.stream)) != hipSuccess) return e;
     for (int i = (int)m.levels.size() - 1; i >= 1; i--) {
         const int lf = m.levels[i];
@@ -110,6 +113,7 @@ int rh_amr_prepare(int levelmin, int levelmax, const int32_t *head, const int32_
     int end = -1;
     std::vector<GodArgs> descs;
     std::vector<int2> items;
+    std::vector<std::array<int, 3>> shapes;  // brick shape of each level's descriptor
     const double gz[3] = {0, 0, 0};
     const double *gg = g ? g : gz;
     for (int l = levelmin; l <= levelmax; l++) {
@@ -124,19 +128,26 @@ int rh_amr_prepare(int levelmin, int levelmax, const int32_t *head, const int32_
         if (int r = fill_args3(h, n, l, dx, gamma, smallr, smallc2, courant_factor, gg, slope, riemann, s.dt_dev, 0.0,
                                a, lb))
             return r;
-        if (!m.levels.empty() && (lb->bx != m.bx || lb->by != m.by || lb->bz != m.bz))
-            return efail(RH_ERR_STATE, "all levels need the same brick shape (rebuild with rh_build_nbor)");
-        m.bx = lb->bx;
-        m.by = lb->by;
-        m.bz = lb->bz;
         a.reflux = l > levelmin ? 1 : 0;
-        const int d = (int)descs.size();
-        for (int b = 0; b < a.nbricks; b++) items.push_back(make_int2(d, b));
+        shapes.push_back({lb->bx, lb->by, lb->bz});
         descs.push_back(a);
         m.levels.push_back(l);
         m.dx_finest = dx;
     }
     if (m.levels.empty()) return efail(RH_ERR_ARG, "no octs in the level range");
+    // items grouped by brick shape: one work-list launch per shape
+    m.groups.clear();
+    for (size_t d = 0; d < descs.size(); d++) {
+        bool seen = false;
+        for (size_t e = 0; e < d; e++) seen = seen || shapes[e] == shapes[d];
+        if (seen) continue;
+        AmrPlan::Group gr{shapes[d][0], shapes[d][1], shapes[d][2], (int)items.size(), 0};
+        for (size_t e = d; e < descs.size(); e++)
+            if (shapes[e] == shapes[d])
+                for (int b = 0; b < descs[e].nbricks; b++) items.push_back(make_int2((int)e, b));
+        gr.count = (int)items.size() - gr.first;
+        m.groups.push_back(gr);
+    }
     m.noct_all = end - m.head0;
     m.levelmin = levelmin;
     m.levelmax = levelmax;
@@ -175,7 +186,6 @@ int rh_amr_prepare(int levelmin, int levelmax, const int32_t *head, const int32_
     RH_HIP_CHECK(dmalloc((void **)&m.d_items, std::max<size_t>(items.size(), 1) * sizeof(int2)));
     RH_HIP_CHECK(hipMemcpyAsync(m.d_descs, both.data(), both.size() * sizeof(GodArgs), hipMemcpyHostToDevice, s.stream));
     RH_HIP_CHECK(hipMemcpyAsync(m.d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, s.stream));
-    for (int r = 0; r <= 8; r++) m.range[r] = (int)(((long)m.nitems * r) / 8);
     // the courant pass writes one partial per block, up to red_cap blocks
     if (int r = ensure_partials(1024)) return r;
     // dt of the first step from the current state; time starts at 0

@@ -24,11 +24,15 @@ struct CourantK {
     int ndim;
 };
 
-// cmpdt, hydro/godunov_utils.f90:228-337, for one cell (no MHD).
+// cmpdt, hydro/godunov_utils.f90:228-337, for one cell (no MHD).  ir = 1/r
+// (the Newton-refined reciprocal) replaces the reference's three divisions by
+// r, and with no magnetic field c_f = sqrt(c2 + sqrt(c2^2)) with c2 = a^2/2 is
+// exactly sqrt(a^2); gravity's correction is a constant when g = 0.
 __device__ __forceinline__ double cell_dt(double uu0, double uu1, double uu2, double uu3, double uu4, const CourantK &c)
 {
     double r = uu0 > c.smallr ? uu0 : c.smallr;
-    double v[3] = {uu1 / r, uu2 / r, uu3 / r};
+    const double ir = rcp_nr(r);
+    double v[3] = {uu1 * ir, uu2 * ir, uu3 * ir};
     double e = uu4;
     e = e - 0.5 * r * (v[0] * v[0]);
     e = e - 0.5 * r * (v[1] * v[1]);
@@ -36,13 +40,15 @@ __device__ __forceinline__ double cell_dt(double uu0, double uu1, double uu2, do
     double pe = (c.gamma - 1.0) * e, pf = r * c.smallp;
     double p = pe > pf ? pe : pf;
     double a2 = c.gamma * p;
-    a2 = a2 / r;
-    double c2 = 0.5 * (0.0 / r + a2);
-    double cf = __builtin_sqrt(c2 + __builtin_sqrt(c2 * c2 - 0.0));
+    a2 = a2 * ir;
+    double cf = __builtin_sqrt(a2);
     double ctot = 0.0;
     for (int d = 0; d < c.ndim; d++) ctot = ctot + __builtin_fabs(v[d]) + cf;
-    double grav = c.gsum_dx / (ctot * ctot);
-    grav = grav > 0.0001 ? grav : 0.0001;
+    double grav = 0.0001;
+    if (c.gsum_dx != 0.0) {
+        grav = c.gsum_dx / (ctot * ctot);
+        grav = grav > 0.0001 ? grav : 0.0001;
+    }
     double dtmax = c.courant_factor * c.dx / c.smallc;
     double dtcell = 2.0 * c.courant_factor * c.dx / (ctot * (1.0 + __builtin_sqrt(1.0 + 2.0 * c.courant_factor * grav)));
     return dtmax < dtcell ? dtmax : dtcell;
@@ -90,10 +96,41 @@ __device__ __forceinline__ void courant_cell(const double u[5], const CourantK &
     ekin = ekin + u[4] * c.vol;
     eint = eint + u[4] * c.vol;
     const double rr = u[0] > c.smallr ? u[0] : c.smallr;
-    eint = eint - 0.5 * (u[1] * u[1]) / rr * c.vol;
-    eint = eint - 0.5 * (u[2] * u[2]) / rr * c.vol;
-    eint = eint - 0.5 * (u[3] * u[3]) / rr * c.vol;
+    const double irr = rcp_nr(rr);
+    eint = eint - 0.5 * (u[1] * u[1]) * irr * c.vol;
+    eint = eint - 0.5 * (u[2] * u[2]) * irr * c.vol;
+    eint = eint - 0.5 * (u[3] * u[3]) * irr * c.vol;
     const double d = cell_dt(u[0], u[1], u[2], u[3], u[4], c);
+    dtmin = d < dtmin ? d : dtmin;
+}
+
+// courant_fine of one cell for the mixed-precision kernel: the conservation
+// sums and the pressure recovery p=(gamma-1)(E-rho v^2/2) in FP64, the signal
+// speed and the cell's dt in FP32 (cmpdt's c_f = sqrt(c2+sqrt(c2^2)) with
+// c2 = a^2/2 and no magnetic field is a), the dt rounded to FP64 for the
+// min-reduction.
+__device__ __forceinline__ void courant_cell_mixed(const double u[5], const CourantK &c, double &mass, double &ekin,
+                                                   double &eint, double &dtmin)
+{
+    mass = mass + u[0] * c.vol;
+    ekin = ekin + u[4] * c.vol;
+    const double rr = u[0] > c.smallr ? u[0] : c.smallr;
+    const double ir = rcp_nr(rr);
+    const double ke = 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) * ir;
+    eint = eint + (u[4] - ke) * c.vol;
+    const double pe = (c.gamma - 1.0) * (u[4] - ke), pf = rr * c.smallp;
+    const double p = pe > pf ? pe : pf;
+    const float irf = (float)ir;
+    const float cf = __builtin_amdgcn_sqrtf((float)(c.gamma * p) * irf);
+    const float ctot = (__builtin_fabsf((float)u[1] * irf) + cf) + (__builtin_fabsf((float)u[2] * irf) + cf) +
+                       (__builtin_fabsf((float)u[3] * irf) + cf);
+    const float cfl = (float)c.courant_factor;
+    float grav = (float)c.gsum_dx * __builtin_amdgcn_rcpf(ctot * ctot);
+    grav = grav > 0.0001f ? grav : 0.0001f;
+    const double dtmax = c.courant_factor * c.dx / c.smallc;
+    const float dtcell = 2.0f * cfl * (float)c.dx *
+                         __builtin_amdgcn_rcpf(ctot * (1.0f + __builtin_amdgcn_sqrtf(1.0f + 2.0f * cfl * grav)));
+    const double d = dtmax < (double)dtcell ? dtmax : (double)dtcell;
     dtmin = d < dtmin ? d : dtmin;
 }
 
@@ -364,6 +401,11 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     const double dtxhalf = dt * 0.5;
     const double gdt[3] = {a.g[0] * dtxhalf, a.g[1] * dtxhalf, a.g[2] * dtxhalf};
     const double inv_dx = a.inv_dx;
+    // face flux times dt/dx in FP64: (F dt)/dx in the reference's order for the
+    // FP64 kernel; F*(dt/dx) with one product for the mixed kernel.  Each face
+    // flux is scaled once and used by both cells, so the update conserves.
+    const double dtdx64 = dt * inv_dx;
+    auto fscale = [&](T f) -> double { return sizeof(T) == 4 ? double(f) * dtdx64 : (double(f) * dt) * inv_dx; };
     const int boxmin[3] = {a.boxmin[0], a.boxmin[1], a.boxmin[2]};
     const int boxmax[3] = {a.boxmax[0], a.boxmax[1], a.boxmax[2]};
     int refl = 0;
@@ -449,7 +491,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 double u[5], q[5];
                 uint8_t rf;
                 gather_cell_m(a.uold, a.refined, v, okey, 3, boxmin, boxmax, refl, c, u, rf);
-                prim_from_cons(u, q, a.smallr, smalle, a.gamma, gdt);
+                prim_from_cons<sizeof(T) == 4>(u, q, a.smallr, smalle, a.gamma, gdt);
 #pragma unroll
                 for (int n = 0; n < 5; n++) sq[st][n][i] = T(q[n]);
                 sref[st][i] = rf;
@@ -460,7 +502,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         } else {
             double q[5];
             cell_flip(u_o, fl_o);
-            prim_from_cons(u_o, q, a.smallr, smalle, a.gamma, gdt);
+            prim_from_cons<sizeof(T) == 4>(u_o, q, a.smallr, smalle, a.gamma, gdt);
 #pragma unroll
             for (int n = 0; n < 5; n++) {
                 qcol_p[n] = T(q[n]);
@@ -469,7 +511,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             sref[st][pc] = rf_o;
             if (tid < NRING) {
                 cell_flip(u_r, fl_r);
-                prim_from_cons(u_r, q, a.smallr, smalle, a.gamma, gdt);
+                prim_from_cons<sizeof(T) == 4>(u_r, q, a.smallr, smalle, a.gamma, gdt);
 #pragma unroll
                 for (int n = 0; n < 5; n++) sq[st][n][ring_i] = T(q[n]);
                 sref[st][ring_i] = rf_r;
@@ -563,7 +605,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             face_flux<RS>(2, qmz_prev, qp[2], f, k);
             const bool z = myref || ref_prev;
 #pragma unroll
-            for (int n = 0; n < 5; n++) fz[n] = z ? 0.0 : (double(f[n]) * dt) * inv_dx;
+            for (int n = 0; n < 5; n++) fz[n] = z ? 0.0 : fscale(f[n]);
         }
         if (a.reflux) {
             // window value and ownership of a cell of the trace region
@@ -577,14 +619,14 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             if (wfx && !(myref || sref[s0][pc - 1])) {
                 const int vl = wv(cx - 1, cy, tt), vh = wv(cx, cy, tt);
 #pragma unroll
-                for (int n = 0; n < 5; n++) F[n] = (double(fx[n]) * dt) * inv_dx;
+                for (int n = 0; n < 5; n++) F[n] = fscale(fx[n]);
                 if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc);
                 if (vh > a.ngridmax && owned(cx - 1, cy, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc);
             }
             if (wfy && !(myref || sref[s0][pc - PX])) {
                 const int vl = wv(cx, cy - 1, tt), vh = wv(cx, cy, tt);
 #pragma unroll
-                for (int n = 0; n < 5; n++) F[n] = (double(fy[n]) * dt) * inv_dx;
+                for (int n = 0; n < 5; n++) F[n] = fscale(fy[n]);
                 if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc);
                 if (vh > a.ngridmax && owned(cx, cy - 1, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc);
             }
@@ -612,10 +654,10 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 const double *pb = a.ubase + (size_t)(v > 0 ? v - 1 : 0) * 40 + ind;
 #pragma unroll
                 for (int n = 0; n < 5; n++) {
-                    const double fl = (double(sf[0][n][tid]) * dt) * inv_dx;
-                    const double fr = (double(sf[0][n][tid + 1]) * dt) * inv_dx;
-                    const double gl = (double(sf[1][n][tid]) * dt) * inv_dx;
-                    const double gr = (double(sf[1][n][tid + CX]) * dt) * inv_dx;
+                    const double fl = fscale(sf[0][n][tid]);
+                    const double fr = fscale(sf[0][n][tid + 1]);
+                    const double gl = fscale(sf[1][n][tid]);
+                    const double gr = fscale(sf[1][n][tid + CX]);
                     part[n] = (pb[8 * n] + (fl - fr)) + (gl - gr);
                 }
             }
@@ -658,7 +700,10 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             if (!own || a.refined[(size_t)(v - 1) * 8 + ind]) continue;
             const double *pu = a.unew + (size_t)(v - 1) * 40 + ind;
             const double un[5] = {pu[0], pu[8], pu[16], pu[24], pu[32]};
-            courant_cell(un, ck, c_mass, c_ekin, c_eint, c_dt);
+            if (sizeof(T) == 4)
+                courant_cell_mixed(un, ck, c_mass, c_ekin, c_eint, c_dt);
+            else
+                courant_cell(un, ck, c_mass, c_ekin, c_eint, c_dt);
         }
         c_mass = wave_sum(c_mass);
         c_ekin = wave_sum(c_ekin);
@@ -815,13 +860,28 @@ __global__ void __launch_bounds__(kCourantThreads) k_courant(const double *__res
     double mass = 0, ekin = 0, eint = 0, dtmin = c.dx / c.smallc;
     const long ncell = (long)noct * ttd;
     const int stride = ttd == 8 ? 40 : 10;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += (long)gridDim.x * blockDim.x) {
-        const long o = (long)(head - 1) + i / ttd;
-        const int ind = (int)(i % ttd);
-        if (refined[o * 8 + ind]) continue;
-        const double *p = uold + o * stride + ind;
-        const double u[5] = {p[0], p[ttd], p[2 * ttd], p[3 * ttd], p[4 * ttd]};
-        courant_cell(u, c, mass, ekin, eint, dtmin);
+    const long step = (long)gridDim.x * blockDim.x;
+    // four cells per thread per round, their loads issued together
+    constexpr int U = 4;
+    for (long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x; i0 < ncell; i0 += U * step) {
+        double u[U][5];
+        bool leaf[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const long i = i0 + k * step;
+            leaf[k] = false;
+            if (i < ncell) {
+                const long o = (long)(head - 1) + i / ttd;
+                const int ind = (int)(i % ttd);
+                const double *p = uold + o * stride + ind;
+#pragma unroll
+                for (int n = 0; n < 5; n++) u[k][n] = p[n * ttd];
+                leaf[k] = !refined[o * 8 + ind];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++)
+            if (leaf[k]) courant_cell(u[k], c, mass, ekin, eint, dtmin);
     }
     __shared__ double red[kCourantThreads / 64][4];
     mass = wave_sum(mass);

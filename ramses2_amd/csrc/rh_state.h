@@ -62,12 +62,16 @@ struct AmrPlan {
     double boxlen = 1.0, dx_finest = 0.0;
     double gamma = 1.4, smallr = 0.0, smallc2 = 0.0, courant = 0.0, g[3] = {0, 0, 0};
     int slope = 0, riemann = 0;
-    int bx = 0, by = 0, bz = 0;
     int ndesc = 0, nitems = 0;
     GodArgs *d_descs = nullptr;    // [2][ndesc]: arrays in both orders
     std::vector<GodArgs> h_descs;  // host copy of d_descs
-    int2 *d_items = nullptr;       // [nitems] (level descriptor, brick)
-    int range[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int2 *d_items = nullptr;       // [nitems] (level descriptor, brick), grouped by brick shape
+    // one work-list launch per brick shape among the levels
+    struct Group {
+        int bx, by, bz;
+        int first, count;          // items [first, first+count) of d_items
+    };
+    std::vector<Group> groups;
     double *racc = nullptr;        // [max(nghost_total,1)][5] reflux sums per ghost oct
     std::vector<int> levels;       // levels with octs, coarse to fine
 };

@@ -98,6 +98,41 @@ module hip_interface
       import c_ptr
     end function rh_last_error
 
+    integer(c_int) function rh_upload(head_idx, num_octs, internal_energy, gamma, smallr, smallc2) &
+         bind(C, name="rh_upload")
+      import c_int, c_double
+      integer(c_int), value :: head_idx, num_octs, internal_energy
+      real(c_double), value :: gamma, smallr, smallc2
+    end function rh_upload
+
+    integer(c_int) function rh_set_interpolation(interpol_var, interpol_type) bind(C, name="rh_set_interpolation")
+      import c_int
+      integer(c_int), value :: interpol_var, interpol_type
+    end function rh_set_interpolation
+
+    ! ---- extensions (include/ramses2_hip_ext.h) ----
+    integer(c_int) function rh_set_precision(precision) bind(C, name="rh_set_precision")
+      import c_int
+      integer(c_int), value :: precision
+    end function rh_set_precision
+
+    integer(c_int) function rh_amr_prepare(levelmin, levelmax, head, noct, boxlen, gamma, smallr, smallc2, &
+         courant_factor, constant_gravity, slope, riemann) bind(C, name="rh_amr_prepare")
+      import c_int, c_double, c_ptr
+      integer(c_int), value :: levelmin, levelmax, slope, riemann
+      type(c_ptr), value :: head, noct
+      real(c_double), value :: boxlen, gamma, smallr, smallc2, courant_factor
+      real(c_double), intent(in) :: constant_gravity(3)
+    end function rh_amr_prepare
+
+    integer(c_int) function rh_amr_run(nsteps, precision, use_graph, dt_last, t_total, sums, ms_per_step) &
+         bind(C, name="rh_amr_run")
+      import c_int, c_double, c_float
+      integer(c_int), value :: nsteps, precision, use_graph
+      real(c_double), intent(out) :: dt_last, t_total, sums(4)
+      real(c_float), intent(out) :: ms_per_step
+    end function rh_amr_run
+
     integer(c_int) function rh_set_brick_shape(bx, by, bz) bind(C, name="rh_set_brick_shape")
       import c_int
       integer(c_int), value :: bx, by, bz

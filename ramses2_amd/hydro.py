@@ -238,7 +238,9 @@ class AmrHierarchy:
     All levels share dt = min over levels (amr_step.f90:588)."""
 
     def __init__(self, amesh, uold, params: HydroParams, ncachemax=None, interpol_var=0, interpol_type=1,
-                 precision=0):
+                 precision=0, bricks=None):
+        """bricks: {level: (bx, by, bz)} brick shape per level (default: the
+        library's current shape for every level)."""
         self.am = amesh
         self.p = params
         self.precision = precision
@@ -252,6 +254,8 @@ class AmrHierarchy:
         self.hh.set_grid_device(uold, uold, amesh.grid_records())
         self.hh.insert_hash_all(1, amesh.ngrid)
         for l in amesh.levels:
+            if bricks is not None:
+                HipHydro.set_brick_shape(*bricks[l])
             self.hh.build_nbor(amesh.head[l], amesh.noct[l], l)
         self.t = 0.0
         self.nstep = 0

@@ -104,3 +104,25 @@ def test_batched_levels_mixed_precision():
     gpu.run_device(1, precision=1)
     err = rel_err_per_var(gpu.state(), om.uold)
     assert np.all(err < 2e-6), err
+
+
+def test_batched_levels_with_per_level_brick_shapes():
+    """Levels built with different brick shapes: one work-list launch per
+    shape inside the batched step, same answer as the oracle."""
+    am = three_level_mesh()
+    u = smooth_state(am, 12, blast=True)
+    om = oracle_mesh(am, u, 0, 2)
+    restrict_all(am, om)
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
+    lo = am.levelmin
+    try:
+        gpu = AmrHierarchy(am, om.uold.copy(), p, interpol_var=0, interpol_type=2,
+                           bricks={lo: (7, 7, 8), lo + 1: (3, 3, 4), lo + 2: (15, 7, 8)})
+        for _ in range(3):
+            oracle_step(am, om, p)
+        gpu.run_device(3, use_graph=True)
+    finally:
+        from ramses2_amd.hydro import HipHydro
+        HipHydro.set_brick_shape(7, 7, 8)
+    err = rel_err_per_var(gpu.state(), om.uold)
+    assert np.all(err < 1e-12), err
