@@ -109,12 +109,16 @@ def cpu_baseline(args, p):
                       % (n, nsteps, el, cpu_model())}
 
 
-def load_traffic():
+def load_traffic(shape):
+    """HBM bytes per launch of the FP64 kernel from the committed counter
+    summary, when it was measured on the brick shape this run uses."""
     path = os.path.join(ROOT, "profiles", "godunov_fp64_turb256_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
+    if d.get("brick_shape") != "%dx%dx%d" % shape:
+        return None
     return d.get("hbm_bytes_per_launch")
 
 
@@ -365,7 +369,7 @@ def main():
     roof = None
     if kms64:
         achieved = BYTES_PER_CELL * ncells / (kms64 * 1e-3) / 1e9
-        traffic = load_traffic()
+        traffic = load_traffic(bricks[0])
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": "k_godunov3d<%d,%d,%d,double> (FP64)" % bricks[0], "kernel_ms": kms64,
