@@ -55,6 +55,7 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="keep the 7x7x8 brick shape (no autotuning)")
     ap.add_argument("--fp64-brick", type=str, default=None, help="brick shape bx,by,bz of the FP64 run (no tuning)")
     ap.add_argument("--mixed-brick", type=str, default=None, help="brick shape of the mixed run (no tuning)")
+    ap.add_argument("--amr-brick", type=str, default=None, help="brick shape of every AMR level (no tuning)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -157,7 +158,9 @@ def run_amr(args):
     cells = int(sum(am.noct[l] for l in am.levels) * 8)
     print("bench: AMR Sedov levels %d-%d, octs per level %s, %d cells (mesh built in %.1f s)"
           % (lmin, lmax, {l: am.noct[l] for l in am.levels}, cells, t_build), file=sys.stderr, flush=True)
-    if args.no_tune:
+    if args.amr_brick:
+        bricks, times = {l: tuple(int(x) for x in args.amr_brick.split(",")) for l in am.levels}, {}
+    elif args.no_tune:
         bricks, times = {l: (7, 7, 8) for l in am.levels}, {}
     else:
         bricks, times = tune_amr(am, u0, p, precision=0, cache=None, key="sedov-amr")

@@ -382,6 +382,7 @@ static void free_level_tables()
     for (auto &kv : st().bricks) {
         dfree(kv.second.win);
         dfree(kv.second.org);
+        dfree(kv.second.zr);
     }
     st().bricks.clear();
     for (auto &kv : st().nbors) {
@@ -773,6 +774,7 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
         if (it != s.bricks.end()) {
             dfree(it->second.win);
             dfree(it->second.org);
+            dfree(it->second.zr);
             s.bricks.erase(it);
         }
         LevelBricks lb;
@@ -793,17 +795,33 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
             const int64_t bz = (s.h_ckey[3 * i + 2] - c.boxmin[2]) / lb.bz;
             return (bz * nby + by) * nbx + bx;
         };
+        std::vector<int32_t> zr;  // per brick: z layers holding octs, zlo | zhi << 16
+        auto zlayer = [&](size_t i) { return (s.h_ckey[3 * i + 2] - c.boxmin[2]) % lb.bz; };
         if (nbx * nby * nbz <= (int64_t(1) << 26)) {
             // occupancy map of the level's brick grid: linear in octs + bricks
-            std::vector<uint8_t> occ((size_t)(nbx * nby * nbz), 0);
-            for (int i = head_idx - 1; i < head_idx - 1 + num_octs; i++) occ[(size_t)brick_of((size_t)i)] = 1;
+            std::vector<int32_t> occ((size_t)(nbx * nby * nbz), -1);
+            for (int i = head_idx - 1; i < head_idx - 1 + num_octs; i++) {
+                int32_t &o = occ[(size_t)brick_of((size_t)i)];
+                const int z = zlayer((size_t)i);
+                o = o < 0 ? (z | z << 16) : (std::min(o & 0xffff, z) | std::max(o >> 16, z) << 16);
+            }
             for (size_t k = 0; k < occ.size(); k++)
-                if (occ[k]) keys.push_back((int64_t)k);
+                if (occ[k] >= 0) {
+                    keys.push_back((int64_t)k);
+                    zr.push_back(occ[k]);
+                }
         } else {
             keys.reserve(num_octs);
             for (int i = head_idx - 1; i < head_idx - 1 + num_octs; i++) keys.push_back(brick_of((size_t)i));
             std::sort(keys.begin(), keys.end());
             keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+            zr.assign(keys.size(), -1);
+            for (int i = head_idx - 1; i < head_idx - 1 + num_octs; i++) {
+                const size_t b = std::lower_bound(keys.begin(), keys.end(), brick_of((size_t)i)) - keys.begin();
+                const int z = zlayer((size_t)i);
+                int32_t &o = zr[b];
+                o = o < 0 ? (z | z << 16) : (std::min(o & 0xffff, z) | std::max(o >> 16, z) << 16);
+            }
         }
         lb.nbricks = (int)keys.size();
         std::vector<int32_t> org(3 * keys.size());
@@ -815,8 +833,10 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
         }
         const int nw = (lb.bx + 2) * (lb.by + 2) * (lb.bz + 2);
         RH_HIP_CHECK(dmalloc((void **)&lb.org, org.size() * sizeof(int32_t)));
+        RH_HIP_CHECK(dmalloc((void **)&lb.zr, zr.size() * sizeof(int32_t)));
         RH_HIP_CHECK(dmalloc((void **)&lb.win, (size_t)lb.nbricks * nw * sizeof(int32_t)));
         RH_HIP_CHECK(hipMemcpyAsync(lb.org, org.data(), org.size() * sizeof(int32_t), hipMemcpyHostToDevice, s.stream));
+        RH_HIP_CHECK(hipMemcpyAsync(lb.zr, zr.data(), zr.size() * sizeof(int32_t), hipMemcpyHostToDevice, s.stream));
         const long n = (long)lb.nbricks * nw;
         hipLaunchKernelGGL(k_build_bricks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s.stream, lb.org,
                            lb.nbricks, lb.bx, lb.by, lb.bz, c, lb.win);
@@ -947,6 +967,7 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         a.refined = s.refined;
         a.win = lb.win;
         a.org = lb.org;
+        a.zr = lb.zr;
         a.nbricks = lb.nbricks;
         a.head = head_idx;
         a.noct = num_subgrids;

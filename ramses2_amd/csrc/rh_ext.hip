@@ -56,6 +56,7 @@ int fill_args3(int head_idx, int num_octs, int ilevel, double dx, double gamma, 
     a.refined = s.refined;
     a.win = lb.win;
     a.org = lb.org;
+    a.zr = lb.zr;
     a.nbricks = lb.nbricks;
     a.head = head_idx;
     a.noct = num_octs;
@@ -170,9 +171,7 @@ hipError_t enqueue_step(int head_idx, int num_octs, double dx, double gamma, dou
         a3->ubase = s.uold;
         a3->courant = 1;
         a3->cpart = s.red_part;
-        if (e0) (void)hipEventRecord(e0, s.stream);
-        hipError_t e = launch_godunov3d(*a3, lb->bx, lb->by, lb->bz, s.precision, s.stream);
-        if (e1) (void)hipEventRecord(e1, s.stream);
+        hipError_t e = launch_godunov3d(*a3, lb->bx, lb->by, lb->bz, s.precision, s.stream, e0, e1);
         if (e != hipSuccess) return e;
         return launch_courant_reduce(s.red_part, lb->nbricks, dx / __builtin_sqrt(smallc2), s.red_out, s.dt_dev,
                                      s.t_dev, s.stream);
@@ -184,19 +183,19 @@ hipError_t enqueue_step(int head_idx, int num_octs, double dx, double gamma, dou
         if (e != hipSuccess) return e;
     }
     hipError_t e;
-    if (e0) (void)hipEventRecord(e0, s.stream);
     if (a3) {
         a3->uold = s.uold;
         a3->unew = s.unew;
         a3->ubase = s.uold;
-        e = launch_godunov3d(*a3, lb->bx, lb->by, lb->bz, s.precision, s.stream);
+        e = launch_godunov3d(*a3, lb->bx, lb->by, lb->bz, s.precision, s.stream, e0, e1);
     } else {
         a1->uold = s.uold;
         a1->unew = s.unew;
         a1->ubase = s.uold;
+        if (e0) (void)hipEventRecord(e0, s.stream);
         e = launch_godunov1d(*a1, s.precision, s.stream);
+        if (e1) (void)hipEventRecord(e1, s.stream);
     }
-    if (e1) (void)hipEventRecord(e1, s.stream);
     return e;
 }
 

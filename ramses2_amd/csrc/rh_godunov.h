@@ -18,6 +18,7 @@ struct GodArgs {
     const uint8_t *refined;   // [oct][8]
     const int32_t *win;       // brick windows
     const int32_t *org;       // brick origins (oct keys)
+    const int32_t *zr;        // per brick: octs present in z layers zlo..zhi, zlo | zhi << 16 (null: all)
     int nbricks;
     int head, noct;           // 1-based active oct range of the level
     int boxmin[3], boxmax[3];
@@ -79,7 +80,8 @@ struct God1Args {
 
 // Launch the 3D brick kernel for brick shape (bx,by,bz) in precision prec
 // (0 = FP64, 1 = FP32 reconstruction/fluxes with FP64 accumulation).
-hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, hipStream_t s);
+hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, hipStream_t s,
+                            hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t launch_godunov1d(const God1Args &a, int prec, hipStream_t s);
 // Work-list launch: items[i] = (descriptor index, brick index), one workgroup per item.
 hipError_t launch_godunov3d_wl(const WorkList &w, int bx, int by, int bz, int slope, int riemann, int prec,

@@ -15,6 +15,8 @@
 //   U: conservative update, accumulated as ((u + dFx) + dFy) + dFz in the
 //      order of godfine1 (hydro/godunov_fine.f90:737-816), written to unew.
 // No MFMA: the work is divides, square roots and compares, not a contraction.
+#include <hip/hip_ext.h>
+
 #include "rh_godunov.h"
 
 namespace rh {
@@ -393,6 +395,10 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     const int tid = threadIdx.x;
     for (int i = tid; i < NW; i += NT) swin[i] = a.win[(size_t)bid * NW + i];
     const int ox0 = a.org[3 * bid + 0], oy0 = a.org[3 * bid + 1], oz0 = a.org[3 * bid + 2];
+    // cell planes [tlo, thi] of the brick hold octs of the level; the march
+    // covers them plus two halo planes on each side
+    const int zr = a.zr ? a.zr[bid] : ((BZ - 1) << 16);
+    const int tlo = 2 * (zr & 0xffff), thi = 2 * (zr >> 16) + 1;
 
     const double dt = a.dt_dev ? *a.dt_dev : a.dt;
     const HydroK<T> k = make_k<T>(dt, a.dx, a.gamma, a.smallr, a.smallc2, a.courant_factor, a.smax_cap, a.use_smax,
@@ -473,11 +479,11 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     uint8_t rf_o, rf_r = 0;
     int fl_o, fl_r = -1;
     if (PF) {
-        fetch(pc, -2, u_o, rf_o, fl_o);
-        if (tid < NRING) fetch(ring_i, -2, u_r, rf_r, fl_r);
+        fetch(pc, tlo - 2, u_o, rf_o, fl_o);
+        if (tid < NRING) fetch(ring_i, tlo - 2, u_r, rf_r, fl_r);
     }
 
-    for (int t = -2; t <= 2 * BZ + 1; ++t) {
+    for (int t = tlo - 2; t <= thi + 2; ++t) {
         const int st = NS == 3 ? ring_slot3(t) : ring_slot2(t);
         // ---- P: primitives of plane t ----
         T qcol_p[5];
@@ -516,13 +522,13 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 for (int n = 0; n < 5; n++) sq[st][n][ring_i] = T(q[n]);
                 sref[st][ring_i] = rf_r;
             }
-            if (t < 2 * BZ + 1) {
+            if (t < thi + 2) {
                 fetch(pc, t + 1, u_o, rf_o, fl_o);
                 if (tid < NRING) fetch(ring_i, t + 1, u_r, rf_r, fl_r);
             }
             __syncthreads();
         }
-        if (t < 0) {
+        if (t < tlo) {
 #pragma unroll
             for (int n = 0; n < 5; n++) {
                 qcol_m[n] = qcol_0[n];
@@ -578,7 +584,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         }
         __syncthreads();
         // ---- F: Riemann problems on the low x, y, z faces of this column ----
-        const bool plane_out = tt >= 0 && tt < 2 * BZ;
+        const bool plane_out = tt >= tlo && tt <= thi;
         T fx[5], fy[5];
         double fz[5] = {0, 0, 0, 0, 0};
         const bool wfx = do_fx && plane_out, wfy = do_fy && plane_out;
@@ -600,7 +606,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
 #pragma unroll
                 for (int n = 0; n < 5; n++) fy[n] = T(0);
         }
-        if (interior && tt >= 0 && tt <= 2 * BZ) {
+        if (interior && tt >= tlo && tt <= thi + 1) {
             T f[5];
             face_flux<RS>(2, qmz_prev, qp[2], f, k);
             const bool z = myref || ref_prev;
@@ -630,7 +636,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc);
                 if (vh > a.ngridmax && owned(cx, cy - 1, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc);
             }
-            if (interior && tt >= 0 && tt <= 2 * BZ && !(myref || ref_prev)) {
+            if (interior && tt >= tlo && tt <= thi + 1 && !(myref || ref_prev)) {
                 const int vl = wv(cx, cy, tt - 1), vh = wv(cx, cy, tt);
                 if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, fz, 8, a.racc);
                 if (vh > a.ngridmax && owned(cx, cy, tt - 1, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, fz, 8, a.racc);
@@ -662,7 +668,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 }
             }
             const int tf = tt - 1;
-            if (tf >= 0 && tf < 2 * BZ) {
+            if (tf >= tlo && tf <= thi) {
                 const int ox = (cx >> 1) + 1, oy = (cy >> 1) + 1, oz = (tf >> 1) + 1;
                 const int v = swin[(oz * WY + oy) * WX + ox];
                 const bool own = v >= a.head && v < a.head + a.noct && ox0 + ox - 1 < boxmax[0] &&
@@ -1054,40 +1060,54 @@ __device__ __forceinline__ void cell_xc(int ndim, int ind, double xc[3])
 
 // interpol_hydro (hydro/interpol_hydro.f90:5-113) with the limiters of
 // :610-752; u1[j][ivar], j = 0 (central parent) and 1..2ndim (-x,+x,-y,+y,-z,+z).
-__device__ void interpol_hydro_dev(int ndim, double u1[7][5], double u2[8][5], int interpol_var, int interpol_type,
+// ND is a compile-time constant so that every loop unrolls and u1/u2 stay in
+// registers (a runtime ndim put them in scratch memory).
+template <int ND>
+__device__ __forceinline__ void interpol_hydro_dev(double u1[7][5], double u2[8][5], int interpol_var, int interpol_type,
                                    double smallr)
 {
-    const int twondim = 2 * ndim, ttd = 1 << ndim;
+    constexpr int ndim = ND, twondim = 2 * ND, ttd = 1 << ND;
     if (interpol_var == 1)
+        #pragma unroll
         for (int j = 0; j <= twondim; j++) {
             double ekin = 0.0;
+            #pragma unroll
             for (int d = 1; d <= 3; d++) ekin = ekin + 0.5 * (u1[j][d] * u1[j][d]) / fmax(u1[j][0], smallr);
             u1[j][4] = u1[j][4] - ekin - 0.0;
         }
+    #pragma unroll
     for (int iv = 0; iv < 5; iv++) {
         double a[7], w[3] = {0, 0, 0};
+        #pragma unroll
         for (int j = 0; j <= twondim; j++) a[j] = u1[j][iv];
         if (interpol_type == 1) {
+            #pragma unroll
             for (int d = 1; d <= ndim; d++) {
                 const double dl = 0.5 * (a[2 * d] - a[0]), dr = 0.5 * (a[0] - a[2 * d - 1]);
                 w[d - 1] = (dl * dr <= 0.0) ? 0.0 : fmin(fabs(dl), fabs(dr)) * dl / fabs(dl);
             }
         } else if (interpol_type == 2 || interpol_type == 3) {
+            #pragma unroll
             for (int d = 1; d <= ndim; d++) w[d - 1] = 0.25 * (a[2 * d] - a[2 * d - 1]);
             if (interpol_type == 2) {
                 double ac[8];
+                #pragma unroll
                 for (int c = 0; c < ttd; c++) ac[c] = a[0];
+                #pragma unroll
                 for (int d = 1; d <= ndim; d++)
+                    #pragma unroll
                     for (int c = 1; c <= ttd; c++) {
                         double xc[3];
                         cell_xc(ndim, c, xc);
                         ac[c - 1] = ac[c - 1] + 2.0 * w[d - 1] * xc[d - 1];
                     }
                 double cmax = ac[0], cmin = ac[0], kmax = a[1], kmin = a[1];
+                #pragma unroll
                 for (int c = 1; c < ttd; c++) {
                     cmax = fmax(cmax, ac[c]);
                     cmin = fmin(cmin, ac[c]);
                 }
+                #pragma unroll
                 for (int j = 2; j <= twondim; j++) {
                     kmax = fmax(kmax, a[j]);
                     kmin = fmin(kmin, a[j]);
@@ -1098,49 +1118,68 @@ __device__ void interpol_hydro_dev(int ndim, double u1[7][5], double u2[8][5], i
                 dc = a[0] - cmin;
                 const double lmin = (dk * dc > 0.0) ? fmin(1.0, dk / dc) : 0.0;
                 const double lim = fmin(lmin, lmax);
+                #pragma unroll
                 for (int d = 0; d < ndim; d++) w[d] = w[d] * lim;
             }
         }
+        #pragma unroll
         for (int c = 1; c <= ttd; c++) {
             double xc[3];
             cell_xc(ndim, c, xc);
             double v = a[0];
+            #pragma unroll
             for (int d = 1; d <= ndim; d++) v = v + w[d - 1] * xc[d - 1];
             u2[c - 1][iv] = v;
         }
     }
     if (interpol_var == 1)
+        #pragma unroll
         for (int c = 0; c < ttd; c++) {
             double ekin = 0.0;
+            #pragma unroll
             for (int d = 1; d <= 3; d++) ekin = ekin + 0.5 * (u2[c][d] * u2[c][d]) / fmax(u2[c][0], smallr);
             u2[c][4] = u2[c][4] + ekin + 0.0;
         }
 }
 
+template <int ND>
 __global__ void __launch_bounds__(128) k_refresh_ghosts(const GhostArgs a)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.nghost) return;
-    const int ttd = 1 << a.ndim, nslot = 2 * a.ndim + 1;
+    constexpr int ttd = 1 << ND, nslot = 2 * ND + 1;
     const int stride = ttd * 5;
     double u1[7][5], u2[8][5];
+    #pragma unroll
     for (int j = 0; j < nslot; j++) {
         const int code = a.src[(g * 7 + j) * 2], ind = a.src[(g * 7 + j) * 2 + 1];
         const int okey[3] = {a.srckey[(g * 7 + j) * 3], a.srckey[(g * 7 + j) * 3 + 1], a.srckey[(g * 7 + j) * 3 + 2]};
         int c[3] = {(ind - 1) & 1, ((ind - 1) >> 1) & 1, ((ind - 1) >> 2) & 1};
         uint8_t rf;
-        gather_cell(a.uold, a.refined, code, okey, a.ndim, a.boxmin, a.boxmax, a.btype, c, u1[j], rf);
+        if constexpr (ND == 3) {
+            // select-based flip: no runtime index into u1 (which would put it in scratch)
+            const CellRef r = cell_ref3(a.uold, a.refined, code, okey, a.boxmin, a.boxmax, reflexive_mask(a.btype), c);
+            cell_load3(r, u1[j], rf);
+            cell_flip(u1[j], r.flip);
+        } else {
+            gather_cell(a.uold, a.refined, code, okey, ND, a.boxmin, a.boxmax, a.btype, c, u1[j], rf);
+        }
     }
-    interpol_hydro_dev(a.ndim, u1, u2, a.interpol_var, a.interpol_type, a.smallr);
+    interpol_hydro_dev<ND>(u1, u2, a.interpol_var, a.interpol_type, a.smallr);
     double *p = a.uold + (size_t)(a.first - 1 + g) * stride;
+    #pragma unroll
     for (int c = 0; c < ttd; c++)
+        #pragma unroll
         for (int n = 0; n < 5; n++) p[n * ttd + c] = u2[c][n];
 }
 
 hipError_t launch_refresh_ghosts(const GhostArgs &a, hipStream_t s)
 {
     if (a.nghost <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_refresh_ghosts, dim3((a.nghost + 127) / 128), dim3(128), 0, s, a);
+    if (a.ndim == 3)
+        hipLaunchKernelGGL(k_refresh_ghosts<3>, dim3((a.nghost + 127) / 128), dim3(128), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_refresh_ghosts<1>, dim3((a.nghost + 127) / 128), dim3(128), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1172,10 +1211,32 @@ __global__ void __launch_bounds__(256) k_upload(double *uold, const int32_t *fat
     }
 }
 
+// upload_fine with interpol_var = 0: one thread per (fine oct, variable),
+// each averaging one contiguous 64-byte line of the oct.
+__global__ void __launch_bounds__(256) k_upload_var(double *uold, const int32_t *__restrict__ father, int head,
+                                                    int noct)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)noct * 5) return;
+    const int io = (int)(i / 5), n = (int)(i % 5);
+    const int igrid = father[2 * io], icell = father[2 * io + 1];
+    if (igrid <= 0) return;
+    const double *c = uold + ((size_t)(head - 1 + io) * 5 + n) * 8;
+    double avg = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) avg = avg + c[k];
+    uold[((size_t)(igrid - 1) * 5 + n) * 8 + (icell - 1)] = avg / 8.0;
+}
+
 hipError_t launch_upload(double *uold, const int32_t *father, int head, int noct, int ndim, int internal_energy,
                          double smallr, hipStream_t s)
 {
     if (noct <= 0) return hipSuccess;
+    if (ndim == 3 && !internal_energy) {
+        const long n = (long)noct * 5;
+        hipLaunchKernelGGL(k_upload_var, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, uold, father, head, noct);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_upload, dim3((noct + 255) / 256), dim3(256), 0, s, uold, father, head, noct, 1 << ndim,
                        internal_energy, smallr);
     return hipGetLastError();
@@ -1203,7 +1264,20 @@ bool brick_shape_supported(int bx, int by, int bz)
     return false;
 }
 
-hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, hipStream_t s)
+// Kernel launch, optionally with start/stop events tracked by the dispatch
+// itself (hipExtLaunchKernelGGL): timing without separate event packets on
+// the stream between the kernels of a step.
+template <typename K, typename... Args>
+static void launch_ev(K kernel, dim3 g, dim3 b, hipStream_t s, hipEvent_t e0, hipEvent_t e1, Args... args)
+{
+    if (e0 || e1)
+        hipExtLaunchKernelGGL(kernel, g, b, 0, s, e0, e1, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, g, b, 0, s, args...);
+}
+
+hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, hipStream_t s, hipEvent_t e0,
+                            hipEvent_t e1)
 {
     if (a.nbricks <= 0) return hipSuccess;
 #define RH_LAUNCH_SHAPE(A, B, C)                                                                         \
@@ -1212,19 +1286,19 @@ hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, 
         const dim3 g(a.nbricks), b(nt);                                                                  \
         if (a.slope == 3) {                                                                              \
             if (prec == 0)                                                                               \
-                hipLaunchKernelGGL((k_godunov3d<A, B, C, double, 0, 3>), g, b, 0, s, a);                 \
+                launch_ev(k_godunov3d<A, B, C, double, 0, 3>, g, b, s, e0, e1, a);                       \
             else                                                                                         \
-                hipLaunchKernelGGL((k_godunov3d<A, B, C, float, 0, 3>), g, b, 0, s, a);                  \
+                launch_ev(k_godunov3d<A, B, C, float, 0, 3>, g, b, s, e0, e1, a);                        \
         } else if (a.riemann == 3) {                                                                     \
             if (prec == 0)                                                                               \
-                hipLaunchKernelGGL((k_godunov3d<A, B, C, double, 3, 2>), g, b, 0, s, a);                 \
+                launch_ev(k_godunov3d<A, B, C, double, 3, 2>, g, b, s, e0, e1, a);                       \
             else                                                                                         \
-                hipLaunchKernelGGL((k_godunov3d<A, B, C, float, 3, 2>), g, b, 0, s, a);                  \
+                launch_ev(k_godunov3d<A, B, C, float, 3, 2>, g, b, s, e0, e1, a);                        \
         } else {                                                                                         \
             if (prec == 0)                                                                               \
-                hipLaunchKernelGGL((k_godunov3d<A, B, C, double, 0, 2>), g, b, 0, s, a);                 \
+                launch_ev(k_godunov3d<A, B, C, double, 0, 2>, g, b, s, e0, e1, a);                       \
             else                                                                                         \
-                hipLaunchKernelGGL((k_godunov3d<A, B, C, float, 0, 2>), g, b, 0, s, a);                  \
+                launch_ev(k_godunov3d<A, B, C, float, 0, 2>, g, b, s, e0, e1, a);                        \
         }                                                                                                \
         return hipGetLastError();                                                                        \
     }

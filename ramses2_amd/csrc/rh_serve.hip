@@ -229,6 +229,7 @@ int rh_ctx_clone_mesh(int src_id)
         const size_t nw = (size_t)(lb.bx + 2) * (lb.by + 2) * (lb.bz + 2);
         RH_HIP_CHECK(dup((void **)&lb.win, kv.second.win, (size_t)lb.nbricks * nw * sizeof(int32_t)));
         RH_HIP_CHECK(dup((void **)&lb.org, kv.second.org, (size_t)lb.nbricks * 3 * sizeof(int32_t)));
+        RH_HIP_CHECK(dup((void **)&lb.zr, kv.second.zr, (size_t)lb.nbricks * sizeof(int32_t)));
         s.bricks[kv.first] = lb;
     }
     return 0;
@@ -267,7 +268,7 @@ int rh_estimate_uniform_bytes(int n_octs_per_dim, int ncachemax, int hash_size, 
     b += 8 * 8 + 2 * 8 + 4 * 4 + 2 * 8;                       // reduction / dt / counters
     b += std::max<int64_t>(4096, nb) * 4 * 8;                 // Courant partials
     b += noct * 27 * 4 + noct * 2 * 4;                        // neighbour and father tables
-    b += nb * ((int64_t)(bx + 2) * (by + 2) * (bz + 2) + 3) * 4;  // brick windows and origins
+    b += nb * ((int64_t)(bx + 2) * (by + 2) * (bz + 2) + 4) * 4;  // brick windows, origins, z ranges
     *bytes = b;
     return 0;
 }

@@ -35,6 +35,7 @@ struct LevelBricks {
     int nbricks = 0;
     int32_t *win = nullptr;   // [nbricks][bz+2][by+2][bx+2]
     int32_t *org = nullptr;   // [nbricks][3] oct key of the brick's first oct
+    int32_t *zr = nullptr;    // [nbricks] z layers holding octs of the level: zlo | zhi << 16
     int boxmin[3] = {0, 0, 0};
     int boxmax[3] = {0, 0, 0};
 };
