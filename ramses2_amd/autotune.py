@@ -123,29 +123,39 @@ def tune_amr(amesh, uold, params, precision=0, steps=4, shapes=None, cache=DEFAU
     return bricks, times
 
 
-def brick_work(shape):
-    """Relative work of one brick: the P phase converts (2BX+4)(2BY+4) cells
-    per plane over 2BZ+4 planes, the T/F/U phases run (2BX+2)(2BY+2) columns
-    over 2BZ+2 planes; a P cell costs about a third of a T/F/U column-plane."""
+def level_work(keys, shape, box_min=(0, 0, 0)):
+    """Modelled work of one level's bricks: a brick on the grid rh_build_nbor
+    uses (origins at box_min + multiples of the shape) marches the planes of
+    its occupied z layers plus two halo planes each side; per plane the P phase
+    converts (2BX+4)(2BY+4) cells and the T/F/U phases run (2BX+2)(2BY+2)
+    columns, a P cell costing about a third of a T/F/U column-plane."""
     bx, by, bz = shape
-    return 0.35 * (2 * bx + 4) * (2 * by + 4) * (2 * bz + 4) + (2 * bx + 2) * (2 * by + 2) * (2 * bz + 2)
+    k = np.asarray(keys, dtype=np.int64) - np.asarray(box_min)
+    b = k // np.asarray(shape)
+    key = b[:, 0] | (b[:, 1] << 21) | (b[:, 2] << 42)
+    order = np.argsort(key, kind="stable")
+    key, z = key[order], (k[:, 2] - b[:, 2] * bz)[order]
+    _, first = np.unique(key, return_index=True)
+    planes = 2 * (np.maximum.reduceat(z, first) - np.minimum.reduceat(z, first) + 1)
+    p_cells = (2 * bx + 4) * (2 * by + 4)
+    columns = (2 * bx + 2) * (2 * by + 2)
+    return float(np.sum(0.35 * p_cells * (planes + 4) + columns * (planes + 2)))
 
 
 def count_bricks(keys, shape, box_min=(0, 0, 0)):
-    """Bricks holding at least one of the octs (oct keys, (n,3)), on the grid
-    rh_build_nbor uses (origins at box_min + multiples of the shape)."""
+    """Bricks holding at least one of the octs (oct keys, (n,3))."""
     k = (np.asarray(keys, dtype=np.int64) - np.asarray(box_min)) // np.asarray(shape)
     return len(np.unique(k[:, 0] | (k[:, 1] << 21) | (k[:, 2] << 42)))
 
 
 def level_shapes(amesh, shapes=None):
-    """Per level, the shape with the least modelled work (bricks x brick_work)."""
+    """Per level, the shape with the least modelled work (level_work)."""
     shapes = shapes or [s for s in SHAPES if s != (1, 1, 1)]
     out = {}
     for l in amesh.levels:
         i0 = amesh.head[l] - 1
         k = amesh.ckey[i0:i0 + amesh.noct[l]]
-        out[l] = min(shapes, key=lambda s: count_bricks(k, s) * brick_work(s))
+        out[l] = min(shapes, key=lambda s: level_work(k, s))
     return out
 
 
