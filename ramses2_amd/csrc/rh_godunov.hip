@@ -363,7 +363,11 @@ __device__ __forceinline__ void reflux_add(double *unew, const int32_t *ghost_pa
 // LDS of one brick workgroup.  NS primitive planes are kept: 2 when the slopes
 // need only face neighbours (the z neighbours of a column sit in registers),
 // 3 for the 27-point slope of slope_type 3.
-template <int BX, int BY, int BZ, typename T, int NS> struct BrickSmem {
+// SEP: face fluxes get their own buffer (sff) instead of overwriting the
+// face states in sf, which saves one barrier per plane.  Measured: no gain
+// (256^3 FP64 and mixed within 1%), so off by default; the barriers are not
+// what limits the kernel.
+template <int BX, int BY, int BZ, typename T, int NS, bool SEP> struct BrickSmem {
     static constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
     static constexpr int PX = 2 * BX + 4, PY = 2 * BY + 4, NP = PX * PY;
     static constexpr int WX = BX + 2, WY = BY + 2, WZ = BZ + 2, NW = WX * WY * WZ;
@@ -371,7 +375,18 @@ template <int BX, int BY, int BZ, typename T, int NS> struct BrickSmem {
     T sq[NS][5][NP];
     uint8_t sref[NS][NP];
     T sf[2][5][NT];
+    T sff[SEP ? 2 : 1][5][SEP ? NT : 1];
 };
+
+// Separate flux buffer where the LDS allows it: the two-plane ring (the
+// 27-point slope's three planes are the big ones) within the 160 KB of a CU.
+#ifndef RH_SEP_FLUX
+#define RH_SEP_FLUX 0
+#endif
+template <int BX, int BY, int BZ, typename T, int NS> constexpr bool sep_flux()
+{
+    return RH_SEP_FLUX && NS == 2 && sizeof(BrickSmem<BX, BY, BZ, T, NS, true>) <= 160 * 1024;
+}
 
 __device__ __forceinline__ int ring_slot2(int t) { return t & 1; }
 __device__ __forceinline__ int ring_slot3(int t) { return (t + 3) % 3; }
@@ -382,7 +397,7 @@ __device__ __forceinline__ int ring_slot3(int t) { return (t + 3) % 3; }
 // (a work-list descriptor): both are read-only for the kernel's lifetime, so
 // the compiler reloads their fields with scalar loads instead of holding them.
 template <int BX, int BY, int BZ, typename T, int RS, int NS, typename A>
-__device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY, BZ, T, NS> &sm)
+__device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY, BZ, T, NS, sep_flux<BX, BY, BZ, T, NS>()> &sm)
 {
     constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
     constexpr int PX = 2 * BX + 4, PY = 2 * BY + 4, NP = PX * PY;
@@ -391,6 +406,12 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     auto &sq = sm.sq;
     auto &sref = sm.sref;
     auto &sf = sm.sf;
+    constexpr bool SEP = sep_flux<BX, BY, BZ, T, NS>();
+    T(*fb)[5][NT];
+    if constexpr (SEP)
+        fb = sm.sff;
+    else
+        fb = sm.sf;
     (void)CY;
     const int tid = threadIdx.x;
     for (int i = tid; i < NW; i += NT) swin[i] = a.win[(size_t)bid * NW + i];
@@ -642,13 +663,13 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 if (vh > a.ngridmax && owned(cx, cy, tt - 1, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, fz, 8, a.racc);
             }
         }
-        __syncthreads();
+        if constexpr (!SEP) __syncthreads();  // the fluxes overwrite the face states in sf
         if (wfx)
 #pragma unroll
-            for (int n = 0; n < 5; n++) sf[0][n][tid] = fx[n];
+            for (int n = 0; n < 5; n++) fb[0][n][tid] = fx[n];
         if (wfy)
 #pragma unroll
-            for (int n = 0; n < 5; n++) sf[1][n][tid] = fy[n];
+            for (int n = 0; n < 5; n++) fb[1][n][tid] = fy[n];
         __syncthreads();
         // ---- U: conservative update ----
         if (interior) {
@@ -660,10 +681,10 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 const double *pb = a.ubase + (size_t)(v > 0 ? v - 1 : 0) * 40 + ind;
 #pragma unroll
                 for (int n = 0; n < 5; n++) {
-                    const double fl = fscale(sf[0][n][tid]);
-                    const double fr = fscale(sf[0][n][tid + 1]);
-                    const double gl = fscale(sf[1][n][tid]);
-                    const double gr = fscale(sf[1][n][tid + CX]);
+                    const double fl = fscale(fb[0][n][tid]);
+                    const double fr = fscale(fb[0][n][tid + 1]);
+                    const double gl = fscale(fb[1][n][tid]);
+                    const double gr = fscale(fb[1][n][tid + CX]);
                     part[n] = (pb[8 * n] + (fl - fr)) + (gl - gr);
                 }
             }
@@ -743,14 +764,18 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
 
 // Minimum waves per SIMD requested from the register allocator for the brick
 // kernels (occupancy vs. register spills; chosen by measurement).
+#ifndef RH_BRICK_MIN_WAVES_F32
+#define RH_BRICK_MIN_WAVES_F32 2
+#endif
 #ifndef RH_BRICK_MIN_WAVES
 #define RH_BRICK_MIN_WAVES 2
 #endif
 
 template <int BX, int BY, int BZ, typename T, int RS, int NS>
-__global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), RH_BRICK_MIN_WAVES) k_godunov3d(const GodArgs a)
+__global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), sizeof(T) == 8 ? RH_BRICK_MIN_WAVES : RH_BRICK_MIN_WAVES_F32)
+    k_godunov3d(const GodArgs a)
 {
-    __shared__ BrickSmem<BX, BY, BZ, T, NS> sm;
+    __shared__ BrickSmem<BX, BY, BZ, T, NS, sep_flux<BX, BY, BZ, T, NS>()> sm;
     // XCD-aware brick assignment: consecutive bricks to the same XCD so that
     // their shared halo octs meet in that XCD's L2.
     int bid = blockIdx.x;
@@ -776,9 +801,10 @@ __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), RH_BRICK_MIN_WAVE
 typedef const __attribute__((address_space(4))) GodArgs ConstGodArgs;
 
 template <int BX, int BY, int BZ, typename T, int RS, int NS>
-__global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), RH_BRICK_MIN_WAVES) k_godunov3d_wl(const WorkList w)
+__global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), sizeof(T) == 8 ? RH_BRICK_MIN_WAVES : RH_BRICK_MIN_WAVES_F32)
+    k_godunov3d_wl(const WorkList w)
 {
-    __shared__ BrickSmem<BX, BY, BZ, T, NS> sm;
+    __shared__ BrickSmem<BX, BY, BZ, T, NS, sep_flux<BX, BY, BZ, T, NS>()> sm;
     const int x = blockIdx.x & 7, l = blockIdx.x >> 3;
     const int i = w.range[x] + l;
     if (i >= w.range[x + 1]) return;
