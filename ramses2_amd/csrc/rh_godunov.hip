@@ -283,7 +283,7 @@ __device__ __forceinline__ HydroK<T> make_k(const double dt, double dx, double g
     HydroK<T> k;
     k.gamma = T(gamma);
     k.smallr = T(smallr);
-    k.smallc2 = T(smallc2);
+    k.smallc2 = T(smallc2 > 0x1p-900 ? smallc2 : 0x1p-900);
     k.smallp_trace = T(smallr * smallc2);
     k.smallp_riem = T(smallc2 / gamma);
     k.entho = T(1.0 / (gamma - 1.0));
@@ -295,7 +295,10 @@ __device__ __forceinline__ HydroK<T> make_k(const double dt, double dx, double g
 }
 
 // Positivity-preserving 27-point slope (slope_type 3, umuscl.f90:2207-2270)
-// for variable n of the cell at ring index (s0, pc).
+// for variable n of the cell at ring index (s0, pc), returned halved like
+// slope_half: the centred differences are formed as (q+ - q-)/4 = dfx/2, so
+// the sum dff and the products slop*dfx/2 are the reference's values halved
+// exactly.
 template <typename T, int PX, int NP>
 __device__ __forceinline__ void slope27(const T (*sq)[5][NP], int sm, int s0, int sp, int n, int pc, T dq3[3])
 
@@ -319,10 +322,10 @@ __device__ __forceinline__ void slope27(const T (*sq)[5][NP], int sm, int s0, in
                     vmax = tmax(vmax, df);
                 }
             }
-    T dfx = T(0.5) * (sq[s0][n][pc + 1] - sq[s0][n][pc - 1]);
-    T dfy = T(0.5) * (sq[s0][n][pc + PX] - sq[s0][n][pc - PX]);
-    T dfz = T(0.5) * (sq[sp][n][pc] - sq[sm][n][pc]);
-    T dff = T(0.5) * (fabs_(dfx) + fabs_(dfy) + fabs_(dfz));
+    T dfx = T(0.25) * (sq[s0][n][pc + 1] - sq[s0][n][pc - 1]);
+    T dfy = T(0.25) * (sq[s0][n][pc + PX] - sq[s0][n][pc - PX]);
+    T dfz = T(0.25) * (sq[sp][n][pc] - sq[sm][n][pc]);
+    T dff = fabs_(dfx) + fabs_(dfy) + fabs_(dfz);
     T slop = dff > T(0) ? tmin(T(1), tdiv(tmin(fabs_(vmin), fabs_(vmax)), dff)) : T(1);
     dq3[0] = slop * dfx;
     dq3[1] = slop * dfy;
@@ -433,6 +436,12 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     // flux is scaled once and used by both cells, so the update conserves.
     const double dtdx64 = dt * inv_dx;
     auto fscale = [&](T f) -> double { return sizeof(T) == 4 ? double(f) * dtdx64 : (double(f) * dt) * inv_dx; };
+    // FP64: the x and y face fluxes are scaled by the thread that solves the
+    // face, before they go to LDS, instead of by each of the two cells that
+    // read them (the same products, half as many); FP32 keeps its fluxes in
+    // FP32 in LDS and scales at the read
+    constexpr bool PRE = sizeof(T) == 8;
+    auto fread = [&](T f) -> double { return PRE ? double(f) : fscale(f); };
     const int boxmin[3] = {a.boxmin[0], a.boxmin[1], a.boxmin[2]};
     const int boxmax[3] = {a.boxmax[0], a.boxmax[1], a.boxmax[2]};
     int refl = 0;
@@ -574,21 +583,30 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 dq[2][n] = d3[2];
             }
         } else {
-            const int sl = k.slope_type;
+            // one uniform branch on the slope type for all 15 slopes
+            auto slopes = [&](const int sl) {
 #pragma unroll
-            for (int n = 0; n < 5; n++) {
-                dq[0][n] = sl ? slope1(sl, sq[s0][n][pc - 1], q0[n], sq[s0][n][pc + 1]) : T(0);
-                dq[1][n] = sl ? slope1(sl, sq[s0][n][pc - PX], q0[n], sq[s0][n][pc + PX]) : T(0);
-                dq[2][n] = sl ? slope1(sl, qcol_m[n], q0[n], qcol_p[n]) : T(0);
-            }
+                for (int n = 0; n < 5; n++) {
+                    dq[0][n] = slope_half(sl, sq[s0][n][pc - 1], q0[n], sq[s0][n][pc + 1]);
+                    dq[1][n] = slope_half(sl, sq[s0][n][pc - PX], q0[n], sq[s0][n][pc + PX]);
+                    dq[2][n] = slope_half(sl, qcol_m[n], q0[n], qcol_p[n]);
+                }
+            };
+            if (k.slope_type == 1)
+                slopes(1);
+            else if (k.slope_type != 0)
+                slopes(2);
+            else
+#pragma unroll
+                for (int n = 0; n < 5; n++) dq[0][n] = dq[1][n] = dq[2][n] = T(0);
         }
         const T inv_r0 = trcp(q0[0]);
         if (k.slope_type > 0) {
-            dq[0][1] = vel_bound(dq[0][1], sq[s0][0][pc - 1], sq[s0][1][pc - 1], inv_r0, q0[1], sq[s0][0][pc + 1],
-                                 sq[s0][1][pc + 1]);
-            dq[1][2] = vel_bound(dq[1][2], sq[s0][0][pc - PX], sq[s0][2][pc - PX], inv_r0, q0[2],
-                                 sq[s0][0][pc + PX], sq[s0][2][pc + PX]);
-            dq[2][3] = vel_bound(dq[2][3], qcol_m[0], qcol_m[3], inv_r0, q0[3], qcol_p[0], qcol_p[3]);
+            dq[0][1] = vel_bound_half(dq[0][1], sq[s0][0][pc - 1], sq[s0][1][pc - 1], inv_r0, q0[1],
+                                      sq[s0][0][pc + 1], sq[s0][1][pc + 1]);
+            dq[1][2] = vel_bound_half(dq[1][2], sq[s0][0][pc - PX], sq[s0][2][pc - PX], inv_r0, q0[2],
+                                      sq[s0][0][pc + PX], sq[s0][2][pc + PX]);
+            dq[2][3] = vel_bound_half(dq[2][3], qcol_m[0], qcol_m[3], inv_r0, q0[3], qcol_p[0], qcol_p[3]);
         }
         T qp[3][5], qm[3][5];
         trace3(q0, dq, qp, qm, k, inv_r0);
@@ -614,18 +632,18 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
 #pragma unroll
             for (int n = 0; n < 5; n++) ql[n] = sf[0][n][tid - 1];
             face_flux<RS>(0, ql, qp[0], fx, k);
-            if (myref || sref[s0][pc - 1])
+            const bool z = myref || sref[s0][pc - 1];
 #pragma unroll
-                for (int n = 0; n < 5; n++) fx[n] = T(0);
+            for (int n = 0; n < 5; n++) fx[n] = z ? T(0) : (PRE ? T(fscale(fx[n])) : fx[n]);
         }
         if (wfy) {
             T ql[5];
 #pragma unroll
             for (int n = 0; n < 5; n++) ql[n] = sf[1][n][tid - CX];
             face_flux<RS>(1, ql, qp[1], fy, k);
-            if (myref || sref[s0][pc - PX])
+            const bool z = myref || sref[s0][pc - PX];
 #pragma unroll
-                for (int n = 0; n < 5; n++) fy[n] = T(0);
+            for (int n = 0; n < 5; n++) fy[n] = z ? T(0) : (PRE ? T(fscale(fy[n])) : fy[n]);
         }
         if (interior && tt >= tlo && tt <= thi + 1) {
             T f[5];
@@ -646,14 +664,14 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             if (wfx && !(myref || sref[s0][pc - 1])) {
                 const int vl = wv(cx - 1, cy, tt), vh = wv(cx, cy, tt);
 #pragma unroll
-                for (int n = 0; n < 5; n++) F[n] = fscale(fx[n]);
+                for (int n = 0; n < 5; n++) F[n] = fread(fx[n]);
                 if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc);
                 if (vh > a.ngridmax && owned(cx - 1, cy, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc);
             }
             if (wfy && !(myref || sref[s0][pc - PX])) {
                 const int vl = wv(cx, cy - 1, tt), vh = wv(cx, cy, tt);
 #pragma unroll
-                for (int n = 0; n < 5; n++) F[n] = fscale(fy[n]);
+                for (int n = 0; n < 5; n++) F[n] = fread(fy[n]);
                 if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc);
                 if (vh > a.ngridmax && owned(cx, cy - 1, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc);
             }
@@ -681,10 +699,10 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 const double *pb = a.ubase + (size_t)(v > 0 ? v - 1 : 0) * 40 + ind;
 #pragma unroll
                 for (int n = 0; n < 5; n++) {
-                    const double fl = fscale(fb[0][n][tid]);
-                    const double fr = fscale(fb[0][n][tid + 1]);
-                    const double gl = fscale(fb[1][n][tid]);
-                    const double gr = fscale(fb[1][n][tid + CX]);
+                    const double fl = fread(fb[0][n][tid]);
+                    const double fr = fread(fb[0][n][tid + 1]);
+                    const double gl = fread(fb[1][n][tid]);
+                    const double gr = fread(fb[1][n][tid + CX]);
                     part[n] = (pb[8 * n] + (fl - fr)) + (gl - gr);
                 }
             }
@@ -853,9 +871,10 @@ __global__ void __launch_bounds__(256) k_godunov1d(const God1Args a)
     for (int c = 1; c <= 3; c++) {
         T dq[5];
 #pragma unroll
-        for (int n = 0; n < 5; n++) dq[n] = k.slope_type ? slope1(k.slope_type, q[c - 1][n], q[c][n], q[c + 1][n]) : T(0);
+        for (int n = 0; n < 5; n++) dq[n] = k.slope_type ? slope_half(k.slope_type, q[c - 1][n], q[c][n], q[c + 1][n]) : T(0);
         const T inv_r = trcp(q[c][0]);
-        if (k.slope_type > 0) dq[1] = vel_bound(dq[1], q[c - 1][0], q[c - 1][1], inv_r, q[c][1], q[c + 1][0], q[c + 1][1]);
+        if (k.slope_type > 0)
+            dq[1] = vel_bound_half(dq[1], q[c - 1][0], q[c - 1][1], inv_r, q[c][1], q[c + 1][0], q[c + 1][1]);
         trace1(q[c], dq, qp[c - 1], qm[c - 1], k, inv_r);
     }
     T fl[5], fr[5];

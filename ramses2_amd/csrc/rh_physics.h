@@ -51,11 +51,33 @@ __device__ __forceinline__ double trcp(double b) { return RH_FAST_F64_DIV ? rcp_
 __device__ __forceinline__ float trcp(float b) { return __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float tsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 
+// Square root of an argument known to lie in [2^-900, 2^1000] (the sound
+// speeds of the Riemann solvers, whose argument is floored by smallc^2, see
+// make_k): the hardware reciprocal square root refined by the Goldschmidt
+// iteration (Markstein, "IA-64 and Elementary Functions", 2000, ch. 9): one
+// coupled step for g ~ sqrt(x) and h ~ 1/(2 sqrt(x)), then two residual
+// corrections d = x - g^2.  Ten instructions; it skips the range scaling and
+// the zero/infinity/NaN fix-ups of the general double sqrt (about 17), which
+// this range never needs, and returns the same correctly rounded results there.
+__device__ __forceinline__ double sqrt_pos(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+__device__ __forceinline__ float sqrt_pos(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Constants of one Godunov launch (hydro_params_t + derived quantities).
 template <typename T> struct HydroK {
     T gamma;
     T smallr;
-    T smallc2;
+    T smallc2;        // smallc**2, floored at 2^-900 (the range of sqrt_pos)
     T smallp_trace;   // smallr*smallc**2       (trace3d, umuscl.f90:934)
     T smallp_riem;    // smallc**2/gamma        (riemann_*, godunov_utils.f90:392)
     T entho;          // one/(gamma-one)
@@ -91,47 +113,56 @@ __device__ __forceinline__ void prim_from_cons(const double u[5], double q[5], d
     q[4] = (gamma - 1.0) * eint;
 }
 
-// uslope, hydro/umuscl.f90:1906-2387: one-dimensional limited slope.
-// slope_type 1 = MinMod (:2132-2170), 2 = MonCen (:2171-2206).
-template <typename T> __device__ __forceinline__ T slope1(int st, T qm1, T q0, T qp1)
+// uslope, hydro/umuscl.f90:1906-2387: one-dimensional limited slope,
+// returned HALVED (dq/2, the quantity the Hancock predictor uses; the halving
+// is exact, so the trace sees the values of the reference's 0.5*dq).
+// slope_type 1 = MinMod (:2132-2170), 2 = MonCen (:2171-2206).  With
+// a = q0-qm1, b = qp1-q0:
+//   MinMod  dq = a, b or 0: the one of smaller magnitude when a*b > 0;
+//   MonCen  dq = dsgn*min(2 min(|a|,|b|), |a+b|/2) when a*b > 0, else 0.
+// Both are formed branch-free as max(min(...), 0) + min(max(...), 0): when
+// all operands share a sign one term is the limited value and the other 0,
+// and when a and b differ in sign both terms are 0.  The values equal the
+// reference's (min/max are exact), apart from the sign of a zero slope and
+// slopes below 1e-154, where the reference's product a*b underflows to 0.
+template <typename T> __device__ __forceinline__ T slope_half(int st, T qm1, T q0, T qp1)
 {
+    const T a = q0 - qm1;
+    const T b = qp1 - q0;
     if (st == 1) {
-        T dlft = q0 - qm1;
-        T drgt = qp1 - q0;
-        if ((dlft * drgt) <= T(0)) return T(0);
-        return dlft > T(0) ? tmin(dlft, drgt) : tmax(dlft, drgt);
+        const T lo = tmax(tmin(a, b), T(0)), hi = tmin(tmax(a, b), T(0));
+        return T(0.5) * (lo + hi);
     }
-    T dlft = T(2) * (q0 - qm1);
-    T drgt = T(2) * (qp1 - q0);
-    T dcen = T(0.5) * (dlft + drgt) * T(0.5);
-    T dsgn = tcopysign(T(1), dcen);
-    T slop = tmin(fabs_(dlft), fabs_(drgt));
-    T dlim = ((dlft * drgt) <= T(0)) ? T(0) : slop;
-    return dsgn * tmin(dlim, fabs_(dcen));
+    const T c = (a + b) * T(0.25);  // dcen/2 = ((2a + 2b)/2)/2/2
+    const T lo = tmax(tmin(tmin(a, b), c), T(0));
+    const T hi = tmin(tmax(tmax(a, b), c), T(0));
+    return lo + hi;
 }
 
-// Normal-velocity slope bound, hydro/umuscl.f90:2361-2385.
+// Normal-velocity slope bound, hydro/umuscl.f90:2361-2385, on a halved slope:
+// |dq| <= 2 min(rho_L |du_L|, rho_R |du_R|)/rho becomes
+// |dq/2| <= min(rho_L |du_L|, rho_R |du_R|)/rho.
 template <typename T>
-__device__ __forceinline__ T vel_bound(T dq, T rl, T ul, T inv_r0, T u0, T rr, T ur)
+__device__ __forceinline__ T vel_bound_half(T dqh, T rl, T ul, T inv_r0, T u0, T rr, T ur)
 {
     T dlft = rl * fabs_(u0 - ul);
     T drgt = rr * fabs_(ur - u0);
-    T dlim = T(2) * tmin(dlft, drgt) * inv_r0;
-    return tcopysign(tmin(fabs_(dq), dlim), dq);
+    T dlimh = tmin(dlft, drgt) * inv_r0;
+    return tcopysign(tmin(fabs_(dqh), dlimh), dqh);
 }
 
 // trace3d, hydro/umuscl.f90:860-1466, hydro part for one cell.
-// q: cell primitives; dq[d][n]: slopes.  Outputs qp[d][n] (right state at the
-// left interface) and qm[d][n] (left state at the right interface).
+// q: cell primitives; dqh[d][n]: halved slopes (the reference's 0.5*dq).
+// Outputs qp[d][n] (right state at the left interface) and qm[d][n] (left
+// state at the right interface).
 template <typename T>
-__device__ __forceinline__ void trace3(const T q[5], const T dq[3][5], T qp[3][5], T qm[3][5], const HydroK<T> &k,
+__device__ __forceinline__ void trace3(const T q[5], const T dqh[3][5], T qp[3][5], T qm[3][5], const HydroK<T> &k,
                                        T inv_r)
 {
-    const T half = T(0.5);
     T r = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
-    T drx = half * dq[0][0], dux = half * dq[0][1], dvx = half * dq[0][2], dwx = half * dq[0][3], dpx = half * dq[0][4];
-    T dry = half * dq[1][0], duy = half * dq[1][1], dvy = half * dq[1][2], dwy = half * dq[1][3], dpy = half * dq[1][4];
-    T drz = half * dq[2][0], duz = half * dq[2][1], dvz = half * dq[2][2], dwz = half * dq[2][3], dpz = half * dq[2][4];
+    T drx = dqh[0][0], dux = dqh[0][1], dvx = dqh[0][2], dwx = dqh[0][3], dpx = dqh[0][4];
+    T dry = dqh[1][0], duy = dqh[1][1], dvy = dqh[1][2], dwy = dqh[1][3], dpy = dqh[1][4];
+    T drz = dqh[2][0], duz = dqh[2][1], dvz = dqh[2][2], dwz = dqh[2][3], dpz = dqh[2][4];
     T divv = dux + dvy + dwz;
     T sr0 = -u * drx - v * dry - w * drz - divv * r;
     T su0 = -u * dux - v * duy - w * duz - dpx * inv_r;
@@ -148,7 +179,7 @@ __device__ __forceinline__ void trace3(const T q[5], const T dq[3][5], T qp[3][5
     for (int d = 0; d < 3; d++) {
 #pragma unroll
         for (int n = 0; n < 5; n++) {
-            T h = half * dq[d][n];
+            T h = dqh[d][n];
             qp[d][n] = ws[n] - h;
             qm[d][n] = ws[n] + h;
         }
@@ -159,13 +190,12 @@ __device__ __forceinline__ void trace3(const T q[5], const T dq[3][5], T qp[3][5
     }
 }
 
-// trace1d, hydro/umuscl.f90:297-481.
+// trace1d, hydro/umuscl.f90:297-481, on halved slopes dqh.
 template <typename T>
-__device__ __forceinline__ void trace1(const T q[5], const T dq[5], T qp[5], T qm[5], const HydroK<T> &k, T inv_r)
+__device__ __forceinline__ void trace1(const T q[5], const T dqh[5], T qp[5], T qm[5], const HydroK<T> &k, T inv_r)
 {
-    const T half = T(0.5);
     T r = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
-    T drx = half * dq[0], dux = half * dq[1], dvx = half * dq[2], dwx = half * dq[3], dpx = half * dq[4];
+    T drx = dqh[0], dux = dqh[1], dvx = dqh[2], dwx = dqh[3], dpx = dqh[4];
     T sr0 = -u * drx - dux * r;
     T sp0 = -u * dpx - dux * k.gamma * p;
     T su0 = -u * dux - dpx * inv_r;
@@ -206,8 +236,8 @@ __device__ __forceinline__ void riemann_hllc(const T ql[5], const T qr[5], T f[5
     T Pr = tmax(qr[4], rr * k.smallp_riem);
     T er = Pr * k.entho;
     T etotr = er + half * rr * (ur * ur + vr * vr + wr * wr);
-    T cfastl = tsqrt(tmax(tdiv(k.gamma * Pl, rl), k.smallc2));
-    T cfastr = tsqrt(tmax(tdiv(k.gamma * Pr, rr), k.smallc2));
+    T cfastl = sqrt_pos(tmax(tdiv(k.gamma * Pl, rl), k.smallc2));
+    T cfastr = sqrt_pos(tmax(tdiv(k.gamma * Pr, rr), k.smallc2));
     T cmax = tmax(cfastl, cfastr);
     T SL = tmin(ul, ur) - cmax;
     T SR = tmax(ul, ur) + cmax;
