@@ -468,23 +468,26 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     // P-phase cells of this thread: its own column's cell, and for the first
     // NRING threads one cell of the ring around the column region.  Their
     // loads are issued one plane ahead (u_o, u_r) and converted the next plane.
-    constexpr int NRING = NP - NT;
+    // The ring leaves out its four corner cells, which no slope reads (the
+    // 27-point stencil reaches them only from the corner columns, whose
+    // states feed no face): 2(CX+CY) cells, one wave for the 7x7 brick.
+    constexpr int NRING = 2 * (CX + CY);
     int ring_i = 0;
     if (tid < NRING) {
         const int r = tid;
         int rx, ry;
-        if (r < PX) {
-            rx = r - 2;
+        if (r < CX) {
+            rx = r - 1;
             ry = -2;
-        } else if (r < 2 * PX) {
-            rx = r - PX - 2;
+        } else if (r < 2 * CX) {
+            rx = r - CX - 1;
             ry = 2 * BY + 1;
-        } else if (r < 2 * PX + 2 * BY + 2) {
+        } else if (r < 2 * CX + CY) {
             rx = -2;
-            ry = r - 2 * PX - 1;
+            ry = r - 2 * CX - 1;
         } else {
             rx = 2 * BX + 1;
-            ry = r - 2 * PX - (2 * BY + 2) - 1;
+            ry = r - 2 * CX - CY - 1;
         }
         ring_i = (ry + 2) * PX + (rx + 2);
     }
@@ -502,9 +505,8 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     // where the registers are free without losing a wave per SIMD, i.e. the
     // FP64 HLLC variant (183 -> 230 of 256); elsewhere the loads are issued at
     // use, as the other variants would spill or drop to fewer waves
-    // (it also needs a thread for every ring cell: not the 1x1x1 brick, 16
-    // threads for 20 ring cells)
-    constexpr bool PF = (sizeof(T) == 8 ? RH_PREFETCH_F64 : RH_PREFETCH_F32) && RS == 3 && NS == 2 && NP - NT <= NT;
+    // (it also needs a thread for every ring cell, NRING <= NT)
+    constexpr bool PF = (sizeof(T) == 8 ? RH_PREFETCH_F64 : RH_PREFETCH_F32) && RS == 3 && NS == 2 && NRING <= NT;
     double u_o[5], u_r[5];
     uint8_t rf_o, rf_r = 0;
     int fl_o, fl_r = -1;
@@ -537,7 +539,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             for (int n = 0; n < 5; n++) qcol_p[n] = sq[st][n][pc];
         } else {
             double q[5];
-            cell_flip(u_o, fl_o);
+            if (fl_o >= 0) cell_flip(u_o, fl_o);  // reflexive ghosts only: skipped by waves without any
             prim_from_cons<sizeof(T) == 4>(u_o, q, a.smallr, smalle, a.gamma, gdt);
 #pragma unroll
             for (int n = 0; n < 5; n++) {
@@ -546,7 +548,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             }
             sref[st][pc] = rf_o;
             if (tid < NRING) {
-                cell_flip(u_r, fl_r);
+                if (fl_r >= 0) cell_flip(u_r, fl_r);
                 prim_from_cons<sizeof(T) == 4>(u_r, q, a.smallr, smalle, a.gamma, gdt);
 #pragma unroll
                 for (int n = 0; n < 5; n++) sq[st][n][ring_i] = T(q[n]);
