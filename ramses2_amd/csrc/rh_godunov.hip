@@ -515,10 +515,9 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         if (tid < NRING) fetch(ring_i, tlo - 2, u_r, rf_r, fl_r);
     }
 
-    for (int t = tlo - 2; t <= thi + 2; ++t) {
+    // ---- P: primitives of plane t into ring slot st, own column to qcol_p ----
+    auto p_phase = [&](const int t, T qcol_p[5]) {
         const int st = NS == 3 ? ring_slot3(t) : ring_slot2(t);
-        // ---- P: primitives of plane t ----
-        T qcol_p[5];
         if constexpr (!PF) {
             for (int i = tid; i < NP; i += NT) {
                 const int px = i % PX - 2, py = i / PX - 2;
@@ -554,22 +553,55 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 for (int n = 0; n < 5; n++) sq[st][n][ring_i] = T(q[n]);
                 sref[st][ring_i] = rf_r;
             }
-            if (t < thi + 2) {
-                fetch(pc, t + 1, u_o, rf_o, fl_o);
-                if (tid < NRING) fetch(ring_i, t + 1, u_r, rf_r, fl_r);
-            }
+            // unconditional (the last plane re-reads itself): a load under a
+            // branch makes the compiler copy the loop-carried registers at the
+            // loop latch, which waits for the loads (and the update's stores)
+            const int tn = t < thi + 2 ? t + 1 : t;
+            // keep the next plane's loads below the conversions that consume
+            // the current values, so that both use the same registers
+            __builtin_amdgcn_sched_barrier(0);
+            fetch(pc, tn, u_o, rf_o, fl_o);
+            if (tid < NRING) fetch(ring_i, tn, u_r, rf_r, fl_r);
             __syncthreads();
         }
-        if (t < tlo) {
+    };
+    auto rotate = [&](const T qcol_p[5]) {
 #pragma unroll
-            for (int n = 0; n < 5; n++) {
-                qcol_m[n] = qcol_0[n];
-                qcol_0[n] = qcol_p[n];
-            }
-            continue;
+        for (int n = 0; n < 5; n++) {
+            qcol_m[n] = qcol_0[n];
+            qcol_0[n] = qcol_p[n];
         }
+    };
+    // the two planes below the first output plane only fill the ring; peeled,
+    // so that the march has one loop body and latch (a skipped body makes the
+    // compiler copy every loop-carried register at the latch, waiting for the
+    // prefetch loads and the update's stores)
+    for (int t = tlo - 2; t < tlo; ++t) {
+        T qcol_p[5];
+        p_phase(t, qcol_p);
+        rotate(qcol_p);
+    }
+    for (int t = tlo; t <= thi + 2; ++t) {
+        const int st = NS == 3 ? ring_slot3(t) : ring_slot2(t);
+        T qcol_p[5];
+        p_phase(t, qcol_p);
         // ---- T: slopes and predictor of plane tt = t-1 ----
         const int tt = t - 1;
+        const bool plane_out = tt >= tlo && tt <= thi;
+        // the values the update of plane tt adds to: with the prefetch (same
+        // register budget), loaded here so that the T and F phases cover
+        // their latency, else at the update
+        double ub[5];
+        if constexpr (PF) {
+            if (interior && plane_out) {
+                const int ox = (cx >> 1) + 1, oy = (cy >> 1) + 1, oz = (tt >> 1) + 1;
+                const int v = swin[(oz * WY + oy) * WX + ox];
+                const int ind = (cx & 1) + 2 * (cy & 1) + 4 * (tt & 1);
+                const double *pb = a.ubase + (size_t)(v > 0 ? v - 1 : 0) * 40 + ind;
+#pragma unroll
+                for (int n = 0; n < 5; n++) ub[n] = pb[8 * n];
+            }
+        }
         const int s0 = NS == 3 ? ring_slot3(t - 1) : ring_slot2(t - 1);
         const int smm = NS == 3 ? ring_slot3(t - 2) : 0;
         T q0[5], dq[3][5];
@@ -615,17 +647,11 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         const uint8_t myref = sref[s0][pc];
 #pragma unroll
         for (int n = 0; n < 5; n++) {
-            qcol_m[n] = qcol_0[n];
-            qcol_0[n] = qcol_p[n];
-        }
-#pragma unroll
-        for (int n = 0; n < 5; n++) {
             sf[0][n][tid] = qm[0][n];
             sf[1][n][tid] = qm[1][n];
         }
         __syncthreads();
         // ---- F: Riemann problems on the low x, y, z faces of this column ----
-        const bool plane_out = tt >= tlo && tt <= thi;
         T fx[5], fy[5];
         double fz[5] = {0, 0, 0, 0, 0};
         const bool wfx = do_fx && plane_out, wfy = do_fy && plane_out;
@@ -695,17 +721,21 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         if (interior) {
             double part[5] = {0, 0, 0, 0, 0};
             if (plane_out) {
-                const int ox = (cx >> 1) + 1, oy = (cy >> 1) + 1, oz = (tt >> 1) + 1;
-                const int v = swin[(oz * WY + oy) * WX + ox];
-                const int ind = (cx & 1) + 2 * (cy & 1) + 4 * (tt & 1);
-                const double *pb = a.ubase + (size_t)(v > 0 ? v - 1 : 0) * 40 + ind;
+                if constexpr (!PF) {
+                    const int ox = (cx >> 1) + 1, oy = (cy >> 1) + 1, oz = (tt >> 1) + 1;
+                    const int v = swin[(oz * WY + oy) * WX + ox];
+                    const int ind = (cx & 1) + 2 * (cy & 1) + 4 * (tt & 1);
+                    const double *pb = a.ubase + (size_t)(v > 0 ? v - 1 : 0) * 40 + ind;
+#pragma unroll
+                    for (int n = 0; n < 5; n++) ub[n] = pb[8 * n];
+                }
 #pragma unroll
                 for (int n = 0; n < 5; n++) {
                     const double fl = fread(fb[0][n][tid]);
                     const double fr = fread(fb[0][n][tid + 1]);
                     const double gl = fread(fb[1][n][tid]);
                     const double gr = fread(fb[1][n][tid + CX]);
-                    part[n] = (pb[8 * n] + (fl - fr)) + (gl - gr);
+                    part[n] = (ub[n] + (fl - fr)) + (gl - gr);
                 }
             }
             const int tf = tt - 1;
@@ -730,6 +760,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
 #pragma unroll
         for (int n = 0; n < 5; n++) qmz_prev[n] = qm[2][n];
         ref_prev = myref;
+        rotate(qcol_p);
     }
     if (a.courant) {
         // courant_fine of the state just written (the dt of the next step):
@@ -1122,7 +1153,7 @@ __device__ __forceinline__ void interpol_hydro_dev(double u1[7][5], double u2[8]
             for (int d = 1; d <= 3; d++) ekin = ekin + 0.5 * (u1[j][d] * u1[j][d]) / fmax(u1[j][0], smallr);
             u1[j][4] = u1[j][4] - ekin - 0.0;
         }
-    #pragma unroll
+#pragma unroll
     for (int iv = 0; iv < 5; iv++) {
         double a[7], w[3] = {0, 0, 0};
         #pragma unroll
@@ -1197,7 +1228,7 @@ __global__ void __launch_bounds__(128) k_refresh_ghosts(const GhostArgs a)
     constexpr int ttd = 1 << ND, nslot = 2 * ND + 1;
     const int stride = ttd * 5;
     double u1[7][5], u2[8][5];
-    #pragma unroll
+#pragma unroll
     for (int j = 0; j < nslot; j++) {
         const int code = a.src[(g * 7 + j) * 2], ind = a.src[(g * 7 + j) * 2 + 1];
         const int okey[3] = {a.srckey[(g * 7 + j) * 3], a.srckey[(g * 7 + j) * 3 + 1], a.srckey[(g * 7 + j) * 3 + 2]};
@@ -1214,7 +1245,7 @@ __global__ void __launch_bounds__(128) k_refresh_ghosts(const GhostArgs a)
     }
     interpol_hydro_dev<ND>(u1, u2, a.interpol_var, a.interpol_type, a.smallr);
     double *p = a.uold + (size_t)(a.first - 1 + g) * stride;
-    #pragma unroll
+#pragma unroll
     for (int c = 0; c < ttd; c++)
         #pragma unroll
         for (int n = 0; n < 5; n++) p[n * ttd + c] = u2[c][n];
