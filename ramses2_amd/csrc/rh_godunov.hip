@@ -26,11 +26,20 @@ struct CourantK {
     int ndim;
 };
 
+// The gravity factor 1 + sqrt(1 + 2 C max(g dx/c_tot^2, 1e-4)) of cmpdt when
+// g = 0: a constant, formed once per thread with the expression cell_dt uses.
+__device__ __forceinline__ double grav_factor0(const CourantK &c)
+{
+    return 1.0 + __builtin_sqrt(1.0 + 2.0 * c.courant_factor * 0.0001);
+}
+
 // cmpdt, hydro/godunov_utils.f90:228-337, for one cell (no MHD).  ir = 1/r
 // (the Newton-refined reciprocal) replaces the reference's three divisions by
 // r, and with no magnetic field c_f = sqrt(c2 + sqrt(c2^2)) with c2 = a^2/2 is
-// exactly sqrt(a^2); gravity's correction is a constant when g = 0.
-__device__ __forceinline__ double cell_dt(double uu0, double uu1, double uu2, double uu3, double uu4, const CourantK &c)
+// exactly sqrt(a^2); gravity's correction is the constant gf0
+// (grav_factor0) when g = 0.
+__device__ __forceinline__ double cell_dt(double uu0, double uu1, double uu2, double uu3, double uu4, const CourantK &c,
+                                          double gf0)
 {
     double r = uu0 > c.smallr ? uu0 : c.smallr;
     const double ir = rcp_nr(r);
@@ -43,16 +52,19 @@ __device__ __forceinline__ double cell_dt(double uu0, double uu1, double uu2, do
     double p = pe > pf ? pe : pf;
     double a2 = c.gamma * p;
     a2 = a2 * ir;
-    double cf = __builtin_sqrt(a2);
+    // a2 >= gamma smallp = smallc^2: the square root of the Riemann solvers
+    // (floored where smallc^2 < 2^-900)
+    double cf = sqrt_pos(a2 > 0x1p-900 ? a2 : 0x1p-900);
     double ctot = 0.0;
     for (int d = 0; d < c.ndim; d++) ctot = ctot + __builtin_fabs(v[d]) + cf;
-    double grav = 0.0001;
+    double gf = gf0;
     if (c.gsum_dx != 0.0) {
-        grav = c.gsum_dx / (ctot * ctot);
+        double grav = c.gsum_dx / (ctot * ctot);
         grav = grav > 0.0001 ? grav : 0.0001;
+        gf = 1.0 + __builtin_sqrt(1.0 + 2.0 * c.courant_factor * grav);
     }
     double dtmax = c.courant_factor * c.dx / c.smallc;
-    double dtcell = 2.0 * c.courant_factor * c.dx / (ctot * (1.0 + __builtin_sqrt(1.0 + 2.0 * c.courant_factor * grav)));
+    double dtcell = 2.0 * c.courant_factor * c.dx / (ctot * gf);
     return dtmax < dtcell ? dtmax : dtcell;
 }
 
@@ -91,8 +103,8 @@ __device__ __forceinline__ CourantK make_courant_k(double dx, int ndim, double g
 }
 
 // One leaf cell of courant_fine (hydro/courant_fine.f90:120-160).
-__device__ __forceinline__ void courant_cell(const double u[5], const CourantK &c, double &mass, double &ekin,
-                                             double &eint, double &dtmin)
+__device__ __forceinline__ void courant_cell(const double u[5], const CourantK &c, double gf0, double &mass,
+                                             double &ekin, double &eint, double &dtmin)
 {
     mass = mass + u[0] * c.vol;
     ekin = ekin + u[4] * c.vol;
@@ -102,7 +114,7 @@ __device__ __forceinline__ void courant_cell(const double u[5], const CourantK &
     eint = eint - 0.5 * (u[1] * u[1]) * irr * c.vol;
     eint = eint - 0.5 * (u[2] * u[2]) * irr * c.vol;
     eint = eint - 0.5 * (u[3] * u[3]) * irr * c.vol;
-    const double d = cell_dt(u[0], u[1], u[2], u[3], u[4], c);
+    const double d = cell_dt(u[0], u[1], u[2], u[3], u[4], c, gf0);
     dtmin = d < dtmin ? d : dtmin;
 }
 
@@ -781,7 +793,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             if (sizeof(T) == 4)
                 courant_cell_mixed(un, ck, c_mass, c_ekin, c_eint, c_dt);
             else
-                courant_cell(un, ck, c_mass, c_ekin, c_eint, c_dt);
+                courant_cell(un, ck, grav_factor0(ck), c_mass, c_ekin, c_eint, c_dt);
         }
         c_mass = wave_sum(c_mass);
         c_ekin = wave_sum(c_ekin);
@@ -937,13 +949,15 @@ __global__ void __launch_bounds__(256) k_godunov1d(const God1Args a)
 // ---------------------------------------------------------------------------
 constexpr int kCourantThreads = 256;
 
+template <int TTD>
 __global__ void __launch_bounds__(kCourantThreads) k_courant(const double *__restrict__ uold,
                                                              const uint8_t *__restrict__ refined, int head, int noct,
-                                                             int ttd, const CourantK c, double *__restrict__ part)
+                                                             const CourantK c, double *__restrict__ part)
 {
     double mass = 0, ekin = 0, eint = 0, dtmin = c.dx / c.smallc;
+    const double gf0 = grav_factor0(c);
+    constexpr int ttd = TTD, stride = 5 * TTD;
     const long ncell = (long)noct * ttd;
-    const int stride = ttd == 8 ? 40 : 10;
     const long step = (long)gridDim.x * blockDim.x;
     // four cells per thread per round, their loads issued together
     constexpr int U = 4;
@@ -965,7 +979,7 @@ __global__ void __launch_bounds__(kCourantThreads) k_courant(const double *__res
         }
 #pragma unroll
         for (int k = 0; k < U; k++)
-            if (leaf[k]) courant_cell(u[k], c, mass, ekin, eint, dtmin);
+            if (leaf[k]) courant_cell(u[k], c, gf0, mass, ekin, eint, dtmin);
     }
     __shared__ double red[kCourantThreads / 64][4];
     mass = wave_sum(mass);
@@ -1005,6 +1019,7 @@ __global__ void __launch_bounds__(kCourantThreads) k_courant_ml(const double *__
                                                                 double *__restrict__ part)
 {
     double mass = 0, ekin = 0, eint = 0, dtmin = c0.dx / c0.smallc;
+    const double gf0 = grav_factor0(c0);
     const long ncell = (long)noct * 8;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += (long)gridDim.x * blockDim.x) {
         const long o = (long)(head - 1) + i / 8;
@@ -1017,7 +1032,7 @@ __global__ void __launch_bounds__(kCourantThreads) k_courant_ml(const double *__
         c.gsum_dx = c0.gsum_dx * dx;
         const double *p = uold + o * 40 + ind;
         const double u[5] = {p[0], p[8], p[16], p[24], p[32]};
-        courant_cell(u, c, mass, ekin, eint, dtmin);
+        courant_cell(u, c, gf0, mass, ekin, eint, dtmin);
     }
     __shared__ double red[kCourantThreads / 64][4];
     mass = wave_sum(mass);
@@ -1515,7 +1530,10 @@ hipError_t launch_courant(const double *uold, const uint8_t *refined, int head, 
     int nb = (int)((ncell + kCourantThreads - 1) / kCourantThreads);
     if (nb > max_blocks) nb = max_blocks;
     if (nb < 1) nb = 1;
-    hipLaunchKernelGGL(k_courant, dim3(nb), dim3(kCourantThreads), 0, s, uold, refined, head, noct, ttd, c, partials);
+    if (ttd == 8)
+        hipLaunchKernelGGL(k_courant<8>, dim3(nb), dim3(kCourantThreads), 0, s, uold, refined, head, noct, c, partials);
+    else
+        hipLaunchKernelGGL(k_courant<2>, dim3(nb), dim3(kCourantThreads), 0, s, uold, refined, head, noct, c, partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(256), 0, s, partials, nb, dx / smallc, out, dt_out, t_acc, 0);
