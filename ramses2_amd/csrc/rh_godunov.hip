@@ -464,7 +464,41 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     const double gvec[3] = {a.g[0], a.g[1], a.g[2]};
     __syncthreads();
 
-    const int cx = tid % CX - 1, cy = tid / CX - 1;
+    // Column of this thread.  Row-major over the (2BX+2)(2BY+2) columns, or,
+    // when the 4 BX BY interior columns fill whole waves and the halo columns
+    // fit one more (HW: the 8x6 brick, 192 + 60 threads), interior columns
+    // first and the halo ring last, so that the halo wave, which needs at most
+    // one face per column and no update, runs one solver pass instead of three
+    // and skips the update; it takes the P-phase ring instead.
+    constexpr int NI = 4 * BX * BY;
+    constexpr bool HW = NI % 64 == 0 && NT - NI <= 64 && RS == 3 && NS == 2;
+    int cx, cy;
+    if constexpr (HW) {
+        if (tid < NI) {
+            cx = tid % (2 * BX);
+            cy = tid / (2 * BX);
+        } else {
+            const int h = tid - NI;
+            if (h < CX) {
+                cx = h - 1;
+                cy = -1;
+            } else if (h < 2 * CX) {
+                cx = h - CX - 1;
+                cy = 2 * BY;
+            } else if (h < 2 * CX + 2 * BY) {
+                cx = -1;
+                cy = h - 2 * CX;
+            } else {
+                cx = 2 * BX;
+                cy = h - 2 * CX - 2 * BY;
+            }
+        }
+    } else {
+        cx = tid % CX - 1;
+        cy = tid / CX - 1;
+    }
+    // index of this column in the face-state and flux buffers (row-major)
+    const int col = (cy + 1) * CX + (cx + 1);
     const int pc = (cy + 2) * PX + (cx + 2);
     const bool interior = cx >= 0 && cx < 2 * BX && cy >= 0 && cy < 2 * BY;
     const bool do_fx = cx >= 0 && cx <= 2 * BX && cy >= 0 && cy < 2 * BY;
@@ -483,12 +517,30 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     // The ring leaves out its four corner cells, which no slope reads (the
     // 27-point stencil reaches them only from the corner columns, whose
     // states feed no face): 2(CX+CY) cells, one wave for the 7x7 brick.
-    constexpr int NRING = 2 * (CX + CY);
+    // With HW the ring is the halo wave's and leaves out the two cells next to
+    // each corner as well, which only the corner columns read with the
+    // one-dimensional slopes of this variant: 4(BX+BY) cells.
+    constexpr int NRING = HW ? 4 * (BX + BY) : 2 * (CX + CY);
+    const int r = HW ? tid - NI : tid;
+    const bool ring = r >= 0 && r < NRING;
     int ring_i = 0;
-    if (tid < NRING) {
-        const int r = tid;
+    if (ring) {
         int rx, ry;
-        if (r < CX) {
+        if constexpr (HW) {
+            if (r < 2 * BX) {
+                rx = r;
+                ry = -2;
+            } else if (r < 4 * BX) {
+                rx = r - 2 * BX;
+                ry = 2 * BY + 1;
+            } else if (r < 4 * BX + 2 * BY) {
+                rx = -2;
+                ry = r - 4 * BX;
+            } else {
+                rx = 2 * BX + 1;
+                ry = r - 4 * BX - 2 * BY;
+            }
+        } else if (r < CX) {
             rx = r - 1;
             ry = -2;
         } else if (r < 2 * CX) {
@@ -517,14 +569,15 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     // where the registers are free without losing a wave per SIMD, i.e. the
     // FP64 HLLC variant (183 -> 230 of 256); elsewhere the loads are issued at
     // use, as the other variants would spill or drop to fewer waves
-    // (it also needs a thread for every ring cell, NRING <= NT)
-    constexpr bool PF = (sizeof(T) == 8 ? RH_PREFETCH_F64 : RH_PREFETCH_F32) && RS == 3 && NS == 2 && NRING <= NT;
+    // (it also needs a thread for every ring cell)
+    constexpr bool PF = (sizeof(T) == 8 ? RH_PREFETCH_F64 : RH_PREFETCH_F32) && RS == 3 && NS == 2 &&
+                        (HW ? NRING <= NT - NI : NRING <= NT);
     double u_o[5], u_r[5];
     uint8_t rf_o, rf_r = 0;
     int fl_o, fl_r = -1;
     if (PF) {
         fetch(pc, tlo - 2, u_o, rf_o, fl_o);
-        if (tid < NRING) fetch(ring_i, tlo - 2, u_r, rf_r, fl_r);
+        if (ring) fetch(ring_i, tlo - 2, u_r, rf_r, fl_r);
     }
 
     // ---- P: primitives of plane t into ring slot st, own column to qcol_p ----
@@ -558,7 +611,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 sq[st][n][pc] = qcol_p[n];
             }
             sref[st][pc] = rf_o;
-            if (tid < NRING) {
+            if (ring) {
                 if (fl_r >= 0) cell_flip(u_r, fl_r);
                 prim_from_cons<sizeof(T) == 4>(u_r, q, a.smallr, smalle, a.gamma, gdt);
 #pragma unroll
@@ -573,7 +626,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             // the current values, so that both use the same registers
             __builtin_amdgcn_sched_barrier(0);
             fetch(pc, tn, u_o, rf_o, fl_o);
-            if (tid < NRING) fetch(ring_i, tn, u_r, rf_r, fl_r);
+            if (ring) fetch(ring_i, tn, u_r, rf_r, fl_r);
             __syncthreads();
         }
     };
@@ -659,31 +712,57 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         const uint8_t myref = sref[s0][pc];
 #pragma unroll
         for (int n = 0; n < 5; n++) {
-            sf[0][n][tid] = qm[0][n];
-            sf[1][n][tid] = qm[1][n];
+            sf[0][n][col] = qm[0][n];
+            sf[1][n][col] = qm[1][n];
         }
         __syncthreads();
         // ---- F: Riemann problems on the low x, y, z faces of this column ----
         T fx[5], fy[5];
         double fz[5] = {0, 0, 0, 0, 0};
         const bool wfx = do_fx && plane_out, wfy = do_fy && plane_out;
-        if (wfx) {
-            T ql[5];
+        if (HW && tid >= NI) {
+            // halo wave: the right column's x face or the top row's y face,
+            // one solver pass with the direction chosen per lane
+            if (wfx || wfy) {
+                const bool dy = wfy;
+                const int nb = dy ? col - CX : col - 1;
+                T ql[5], qr[5], a5[5], b5[5], g[5];
 #pragma unroll
-            for (int n = 0; n < 5; n++) ql[n] = sf[0][n][tid - 1];
-            face_flux<RS>(0, ql, qp[0], fx, k);
-            const bool z = myref || sref[s0][pc - 1];
+                for (int n = 0; n < 5; n++) {
+                    ql[n] = dy ? sf[1][n][nb] : sf[0][n][nb];
+                    qr[n] = dy ? qp[1][n] : qp[0][n];
+                }
+                a5[0] = ql[0], a5[1] = dy ? ql[2] : ql[1], a5[2] = dy ? ql[1] : ql[2], a5[3] = ql[3], a5[4] = ql[4];
+                b5[0] = qr[0], b5[1] = dy ? qr[2] : qr[1], b5[2] = dy ? qr[1] : qr[2], b5[3] = qr[3], b5[4] = qr[4];
+                riemann_hllc(a5, b5, g, k);
+                const bool z = myref || sref[s0][dy ? pc - PX : pc - 1];
+                T f[5] = {g[0], dy ? g[2] : g[1], dy ? g[1] : g[2], g[3], g[4]};
 #pragma unroll
-            for (int n = 0; n < 5; n++) fx[n] = z ? T(0) : (PRE ? T(fscale(fx[n])) : fx[n]);
-        }
-        if (wfy) {
-            T ql[5];
+                for (int n = 0; n < 5; n++) {
+                    f[n] = z ? T(0) : (PRE ? T(fscale(f[n])) : f[n]);
+                    fx[n] = f[n];
+                    fy[n] = f[n];
+                }
+            }
+        } else {
+            if (wfx) {
+                T ql[5];
 #pragma unroll
-            for (int n = 0; n < 5; n++) ql[n] = sf[1][n][tid - CX];
-            face_flux<RS>(1, ql, qp[1], fy, k);
-            const bool z = myref || sref[s0][pc - PX];
+                for (int n = 0; n < 5; n++) ql[n] = sf[0][n][col - 1];
+                face_flux<RS>(0, ql, qp[0], fx, k);
+                const bool z = myref || sref[s0][pc - 1];
 #pragma unroll
-            for (int n = 0; n < 5; n++) fy[n] = z ? T(0) : (PRE ? T(fscale(fy[n])) : fy[n]);
+                for (int n = 0; n < 5; n++) fx[n] = z ? T(0) : (PRE ? T(fscale(fx[n])) : fx[n]);
+            }
+            if (wfy) {
+                T ql[5];
+#pragma unroll
+                for (int n = 0; n < 5; n++) ql[n] = sf[1][n][col - CX];
+                face_flux<RS>(1, ql, qp[1], fy, k);
+                const bool z = myref || sref[s0][pc - PX];
+#pragma unroll
+                for (int n = 0; n < 5; n++) fy[n] = z ? T(0) : (PRE ? T(fscale(fy[n])) : fy[n]);
+            }
         }
         if (interior && tt >= tlo && tt <= thi + 1) {
             T f[5];
@@ -724,10 +803,10 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         if constexpr (!SEP) __syncthreads();  // the fluxes overwrite the face states in sf
         if (wfx)
 #pragma unroll
-            for (int n = 0; n < 5; n++) fb[0][n][tid] = fx[n];
+            for (int n = 0; n < 5; n++) fb[0][n][col] = fx[n];
         if (wfy)
 #pragma unroll
-            for (int n = 0; n < 5; n++) fb[1][n][tid] = fy[n];
+            for (int n = 0; n < 5; n++) fb[1][n][col] = fy[n];
         __syncthreads();
         // ---- U: conservative update ----
         if (interior) {
@@ -743,10 +822,10 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 }
 #pragma unroll
                 for (int n = 0; n < 5; n++) {
-                    const double fl = fread(fb[0][n][tid]);
-                    const double fr = fread(fb[0][n][tid + 1]);
-                    const double gl = fread(fb[1][n][tid]);
-                    const double gr = fread(fb[1][n][tid + CX]);
+                    const double fl = fread(fb[0][n][col]);
+                    const double fr = fread(fb[0][n][col + 1]);
+                    const double gl = fread(fb[1][n][col]);
+                    const double gr = fread(fb[1][n][col + CX]);
                     part[n] = (ub[n] + (fl - fr)) + (gl - gr);
                 }
             }
@@ -1341,6 +1420,8 @@ hipError_t launch_upload(double *uold, const int32_t *father, int head, int noct
 #define RH_BRICK_SHAPES(X) \
     X(7, 7, 8)             \
     X(7, 7, 16)            \
+    X(8, 6, 8)             \
+    X(8, 6, 16)            \
     X(15, 7, 8)            \
     X(15, 7, 16)           \
     X(7, 7, 4)             \

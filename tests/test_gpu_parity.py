@@ -60,8 +60,8 @@ def test_godunov3d_floors_fp64():
 
 
 @pytest.mark.parametrize("variant", [(2, "hllc"), (3, "llf"), (1, "hll")])
-@pytest.mark.parametrize("brick", [(7, 7, 8), (7, 7, 16), (15, 7, 8), (15, 7, 16), (3, 3, 4), (1, 1, 1), (3, 3, 8),
-                                   (7, 7, 4)])
+@pytest.mark.parametrize("brick", [(7, 7, 8), (7, 7, 16), (8, 6, 8), (8, 6, 16), (15, 7, 8), (15, 7, 16), (3, 3, 4),
+                                   (1, 1, 1), (3, 3, 8), (7, 7, 4)])
 def test_brick_shapes_agree(brick, variant):
     """Every compiled brick shape, in each kernel variant (HLLC; the 27-point
     slope's three-plane ring; the generic solver path), gives the oracle's
