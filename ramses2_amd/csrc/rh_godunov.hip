@@ -83,6 +83,31 @@ __device__ __forceinline__ double wave_min(double x)
     }
     return x;
 }
+// The same for a wave of which only lanes [0, nvalid) exist (the last wave of
+// a workgroup whose size is not a multiple of 64): partners at or beyond
+// nvalid are skipped.  Lane 0's partners in the round of offset o are lanes
+// o..2o-1, whose partial results only gather lanes above them, so lane 0
+// ends with the reduction over exactly the existing lanes.
+__device__ __forceinline__ double wave_sum_n(double x, int nvalid)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double y = __shfl_xor(x, o, 64);
+        if ((lane ^ o) < nvalid) x += y;
+    }
+    return x;
+}
+__device__ __forceinline__ double wave_min_n(double x, int nvalid)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double y = __shfl_xor(x, o, 64);
+        if ((lane ^ o) < nvalid && y < x) x = y;
+    }
+    return x;
+}
 
 __device__ __forceinline__ CourantK make_courant_k(double dx, int ndim, double gamma, double smallr, double smallc2,
                                                    double courant_factor, const double g[3])
@@ -874,10 +899,12 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             else
                 courant_cell(un, ck, grav_factor0(ck), c_mass, c_ekin, c_eint, c_dt);
         }
-        c_mass = wave_sum(c_mass);
-        c_ekin = wave_sum(c_ekin);
-        c_eint = wave_sum(c_eint);
-        c_dt = wave_min(c_dt);
+        // lanes of this wave that exist (NT need not be a multiple of 64)
+        const int nv = NT - (tid & ~63) < 64 ? NT - (tid & ~63) : 64;
+        c_mass = wave_sum_n(c_mass, nv);
+        c_ekin = wave_sum_n(c_ekin, nv);
+        c_eint = wave_sum_n(c_eint, nv);
+        c_dt = wave_min_n(c_dt, nv);
         __syncthreads();
         double *red = (double *)&sf[0][0][0];
         const int w = tid >> 6;
@@ -1422,6 +1449,8 @@ hipError_t launch_upload(double *uold, const int32_t *father, int head, int noct
     X(7, 7, 16)            \
     X(8, 6, 8)             \
     X(8, 6, 16)            \
+    X(4, 4, 4)             \
+    X(4, 4, 8)             \
     X(15, 7, 8)            \
     X(15, 7, 16)           \
     X(7, 7, 4)             \

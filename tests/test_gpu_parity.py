@@ -59,9 +59,12 @@ def test_godunov3d_floors_fp64():
     assert np.all(err < 1e-10), err
 
 
+BRICKS = [(7, 7, 8), (7, 7, 16), (8, 6, 8), (8, 6, 16), (4, 4, 4), (4, 4, 8), (15, 7, 8), (15, 7, 16), (3, 3, 4),
+          (1, 1, 1), (3, 3, 8), (7, 7, 4)]
+
+
 @pytest.mark.parametrize("variant", [(2, "hllc"), (3, "llf"), (1, "hll")])
-@pytest.mark.parametrize("brick", [(7, 7, 8), (7, 7, 16), (8, 6, 8), (8, 6, 16), (15, 7, 8), (15, 7, 16), (3, 3, 4),
-                                   (1, 1, 1), (3, 3, 8), (7, 7, 4)])
+@pytest.mark.parametrize("brick", BRICKS)
 def test_brick_shapes_agree(brick, variant):
     """Every compiled brick shape, in each kernel variant (HLLC; the 27-point
     slope's three-plane ring; the generic solver path), gives the oracle's
@@ -75,6 +78,38 @@ def test_brick_shapes_agree(brick, variant):
         from ramses2_amd.hydro import HipHydro
         HipHydro.set_brick_shape(7, 7, 8)
     assert np.all(rel_err_per_var(g, o) < TOL64)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("brick", BRICKS)
+def test_fused_courant_every_shape(brick, precision):
+    """The Courant reduction in the Godunov kernel's epilogue, for every brick
+    shape (the 8x6, 4x4 and 1x1x1 workgroups end in a partial wave), steps with
+    the dts of the separate Courant pass: the same time and state, bit for bit
+    in FP64 (dt is a min, exact in any order), to the mixed tolerance in FP32
+    (the epilogue's signal speed is FP32 there).  The returned dt differs by
+    design: the separate pass reports the last step's dt, the epilogue the
+    next one's; the conserved sums (mass, energy) agree."""
+    from ramses2_amd.hydro import HipHydro
+    mesh = UniformMesh(3, 4)
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
+    cells = ic.sedov_3d(16)
+    HipHydro.set_brick_shape(*brick)
+    try:
+        a = UniformLevel(mesh, cells, p)
+        dta, ta, sa, _ = a.run_device(3, precision=precision, fused_courant=0)
+        b = UniformLevel(mesh, cells, p)
+        dtb, tb, sb, _ = b.run_device(3, precision=precision, fused_courant=1)
+    finally:
+        HipHydro.set_brick_shape(7, 7, 8)
+    assert dta > 0 and dtb > 0 and np.isfinite(dtb) and abs(dtb - dta) < 0.05 * dta
+    if precision == 0:
+        assert tb == ta
+        assert np.array_equal(b.state(), a.state())
+    else:
+        assert abs(tb - ta) <= 1e-5 * ta
+        assert np.all(rel_err_per_var(b.state(), a.state()) < 1e-5)
+    assert np.allclose(sb[:2], sa[:2], rtol=1e-12, atol=0)
 
 
 def test_courant_fp64():
