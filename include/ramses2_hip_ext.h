@@ -20,9 +20,10 @@ extern "C" {
 int rh_set_precision(int precision);
 
 /* Courant pass of the next step inside the Godunov kernel (rh_level_run, 3D):
- * -1 automatic (on for precision 1 only, where it measures faster), 0 a
- * separate Courant kernel before each step, 1 the epilogue of the Godunov
- * kernel reduces the Courant condition of the state it has just written. */
+ * -1 automatic (on: it measures faster in both precisions), 0 a separate
+ * Courant kernel before each step, 1 the Godunov kernel reduces the Courant
+ * condition of the state it writes (the FP64 kernel from each cell's new state
+ * as it stores it, the FP32 kernel in an epilogue that re-reads its cells). */
 int rh_set_fused_courant(int mode);
 
 /* Oct-brick shape (octs per brick in x, y, z) used by the next rh_build_nbor

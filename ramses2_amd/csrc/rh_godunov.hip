@@ -671,6 +671,14 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         p_phase(t, qcol_p);
         rotate(qcol_p);
     }
+    // courant_fine of the state written (the dt of the next step), when asked
+    // for: the FP64 kernel takes each leaf cell's new state as it stores it;
+    // the FP32 kernel, one wave per SIMD short of registers for that, re-reads
+    // the brick's cells after the march
+    constexpr bool CU = sizeof(T) == 8;
+    const CourantK ck = make_courant_k(a.dx, 3, a.gamma, a.smallr, a.smallc2, a.courant_factor, gvec);
+    const double gf0 = grav_factor0(ck);
+    double c_mass = 0.0, c_ekin = 0.0, c_eint = 0.0, c_dt = a.dx / ck.smallc;
     for (int t = tlo; t <= thi + 2; ++t) {
         const int st = NS == 3 ? ring_slot3(t) : ring_slot2(t);
         T qcol_p[5];
@@ -863,8 +871,13 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 if (own) {
                     const int ind = (cx & 1) + 2 * (cy & 1) + 4 * (tf & 1);
                     double *po = a.unew + (size_t)(v - 1) * 40 + ind;
+                    double un[5];
 #pragma unroll
-                    for (int n = 0; n < 5; n++) po[8 * n] = part_prev[n] + (fz_prev[n] - fz[n]);
+                    for (int n = 0; n < 5; n++) {
+                        un[n] = part_prev[n] + (fz_prev[n] - fz[n]);
+                        po[8 * n] = un[n];
+                    }
+                    if (CU && a.courant && !ref_prev) courant_cell(un, ck, gf0, c_mass, c_ekin, c_eint, c_dt);
                 }
             }
 #pragma unroll
@@ -879,13 +892,11 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         rotate(qcol_p);
     }
     if (a.courant) {
-        // courant_fine of the state just written (the dt of the next step):
-        // the brick's own leaf cells, re-read from unew while still in cache,
-        // then the brick's partials in a fixed reduction order (FP64)
-        const CourantK ck = make_courant_k(a.dx, 3, a.gamma, a.smallr, a.smallc2, a.courant_factor, gvec);
-        double c_mass = 0.0, c_ekin = 0.0, c_eint = 0.0, c_dt = a.dx / ck.smallc;
+        // the brick's partials of courant_fine in a fixed reduction order
+        // (FP64); the FP32 kernel first re-reads its own leaf cells from unew
+        // while still in cache
         __syncthreads();
-        for (int i = tid; i < BX * BY * BZ * 8; i += NT) {
+        for (int i = tid; !CU && i < BX * BY * BZ * 8; i += NT) {
             const int oc = i >> 3, ind = i & 7;
             const int ox = oc % BX + 1, oy = (oc / BX) % BY + 1, oz = oc / (BX * BY) + 1;
             const int v = swin[(oz * WY + oy) * WX + ox];
@@ -894,10 +905,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             if (!own || a.refined[(size_t)(v - 1) * 8 + ind]) continue;
             const double *pu = a.unew + (size_t)(v - 1) * 40 + ind;
             const double un[5] = {pu[0], pu[8], pu[16], pu[24], pu[32]};
-            if (sizeof(T) == 4)
-                courant_cell_mixed(un, ck, c_mass, c_ekin, c_eint, c_dt);
-            else
-                courant_cell(un, ck, grav_factor0(ck), c_mass, c_ekin, c_eint, c_dt);
+            courant_cell_mixed(un, ck, c_mass, c_ekin, c_eint, c_dt);
         }
         // lanes of this wave that exist (NT need not be a multiple of 64)
         const int nv = NT - (tid & ~63) < 64 ? NT - (tid & ~63) : 64;
