@@ -418,6 +418,15 @@ template <int BX, int BY, int BZ, typename T, int NS, bool SEP> struct BrickSmem
     T sff[SEP ? 2 : 1][5][SEP ? NT : 1];
 };
 
+// The FP32 per-level kernel's Courant reduction in the update phase (as the
+// FP64 kernel does) instead of re-reading the brick after the march: it costs
+// the kernel a wave per SIMD, which the uniform level step recovers (step
+// 0.93 -> 0.87 ms at 256^3) and the work-list kernel of the AMR and serving
+// steps does not (AMR mixed step 5% slower), so the latter keeps the re-read.
+#ifndef RH_COURANT_IN_UPDATE_F32
+#define RH_COURANT_IN_UPDATE_F32 1
+#endif
+
 // Separate flux buffer where the LDS allows it: the two-plane ring (the
 // 27-point slope's three planes are the big ones) within the 160 KB of a CU.
 #ifndef RH_SEP_FLUX
@@ -436,7 +445,7 @@ __device__ __forceinline__ int ring_slot3(int t) { return (t + 3) % 3; }
 // A is GodArgs (a kernel argument) or GodArgs in the constant address space
 // (a work-list descriptor): both are read-only for the kernel's lifetime, so
 // the compiler reloads their fields with scalar loads instead of holding them.
-template <int BX, int BY, int BZ, typename T, int RS, int NS, typename A>
+template <int BX, int BY, int BZ, typename T, int RS, int NS, bool WL, typename A>
 __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY, BZ, T, NS, sep_flux<BX, BY, BZ, T, NS>()> &sm)
 {
     constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
@@ -672,10 +681,10 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         rotate(qcol_p);
     }
     // courant_fine of the state written (the dt of the next step), when asked
-    // for: the FP64 kernel takes each leaf cell's new state as it stores it;
-    // the FP32 kernel, one wave per SIMD short of registers for that, re-reads
-    // the brick's cells after the march
-    constexpr bool CU = sizeof(T) == 8;
+    // for: the FP64 kernel and the FP32 per-level kernel take each leaf
+    // cell's new state as they store it; the FP32 work-list kernel re-reads
+    // the brick's cells after the march (RH_COURANT_IN_UPDATE_F32)
+    constexpr bool CU = sizeof(T) == 8 || (RH_COURANT_IN_UPDATE_F32 && !WL);
     const CourantK ck = make_courant_k(a.dx, 3, a.gamma, a.smallr, a.smallc2, a.courant_factor, gvec);
     const double gf0 = grav_factor0(ck);
     double c_mass = 0.0, c_ekin = 0.0, c_eint = 0.0, c_dt = a.dx / ck.smallc;
@@ -877,7 +886,12 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                         un[n] = part_prev[n] + (fz_prev[n] - fz[n]);
                         po[8 * n] = un[n];
                     }
-                    if (CU && a.courant && !ref_prev) courant_cell(un, ck, gf0, c_mass, c_ekin, c_eint, c_dt);
+                    if (CU && a.courant && !ref_prev) {
+                        if constexpr (sizeof(T) == 4)
+                            courant_cell_mixed(un, ck, c_mass, c_ekin, c_eint, c_dt);
+                        else
+                            courant_cell(un, ck, gf0, c_mass, c_ekin, c_eint, c_dt);
+                    }
                 }
             }
 #pragma unroll
@@ -962,7 +976,7 @@ __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), sizeof(T) == 8 ? 
         bid = x * q + (x < r ? x : r) + l;
     }
     if (bid >= a.nbricks) return;
-    brick_body<BX, BY, BZ, T, RS, NS>(a, bid, sm);
+    brick_body<BX, BY, BZ, T, RS, NS, false>(a, bid, sm);
 }
 
 // Work-list kernel: work item i = (descriptor, brick), where a descriptor
@@ -987,7 +1001,7 @@ __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), sizeof(T) == 8 ? 
     if (i >= w.range[x + 1]) return;
     const int2 it = w.items[i];
     const int di = __builtin_amdgcn_readfirstlane(it.x), bi = __builtin_amdgcn_readfirstlane(it.y);
-    brick_body<BX, BY, BZ, T, RS, NS>(((ConstGodArgs *)w.descs)[di], bi, sm);
+    brick_body<BX, BY, BZ, T, RS, NS, true>(((ConstGodArgs *)w.descs)[di], bi, sm);
 }
 
 
