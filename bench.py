@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--amr-steps", type=int, default=20)
     ap.add_argument("--amr-levels", type=str, default="6,10", help="levelmin,levelmax of the AMR Sedov case")
     ap.add_argument("--no-amr", action="store_true")
-    ap.add_argument("--no-tune", action="store_true", help="keep the 7x7x8 brick shape (no autotuning)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the library's default 8x6x8 brick shape (no autotuning)")
     ap.add_argument("--fp64-brick", type=str, default=None, help="brick shape bx,by,bz of the FP64 run (no tuning)")
     ap.add_argument("--mixed-brick", type=str, default=None, help="brick shape of the mixed run (no tuning)")
     ap.add_argument("--amr-brick", type=str, default=None, help="brick shape of every AMR level (no tuning)")
@@ -145,6 +145,7 @@ def run_amr(args):
     gradients > 0.1, MonCen, HLLC, C 0.5): the batched step (every level in one
     work-list launch, hipGraph-replayed, brick shape from the autotuner) in FP64
     and mixed precision, beside the per-level drop-in call sequence."""
+    from ramses2_amd.autotune import DEFAULT_SHAPE
     from ramses2_amd import HydroParams
     from ramses2_amd.amr_build import sedov_amr
     from ramses2_amd.autotune import tune_amr
@@ -161,7 +162,7 @@ def run_amr(args):
     if args.amr_brick:
         bricks, times = {l: tuple(int(x) for x in args.amr_brick.split(",")) for l in am.levels}, {}
     elif args.no_tune:
-        bricks, times = {l: (7, 7, 8) for l in am.levels}, {}
+        bricks, times = {l: DEFAULT_SHAPE for l in am.levels}, {}
     else:
         bricks, times = tune_amr(am, u0, p, precision=0, cache=None, key="sedov-amr")
     print("bench: AMR brick shapes (ms/step): %s -> %s" % ({k: round(v, 3) for k, v in times.items()}, bricks),
@@ -208,7 +209,7 @@ def run_amr(args):
     out["fp64"]["speedup_vs_dropin"] = ms_d / out["fp64"]["ms_per_step"]
     print("bench: AMR drop-in per level %.3f ms/step; batched FP64 x%.2f, mixed x%.2f vs FP64"
           % (ms_d, out["fp64"]["speedup_vs_dropin"], out["mixed"]["speedup_vs_fp64"]), file=sys.stderr, flush=True)
-    HipHydro.set_brick_shape(7, 7, 8)
+    HipHydro.set_brick_shape(*DEFAULT_SHAPE)
     return out
 
 
@@ -216,6 +217,7 @@ def run_serving(args):
     """Ensemble serving: capacity of the batched FP64 step measured on a closed
     batch of the stream's first jobs, then the Poisson stream of all jobs at
     serve_load of that capacity, served in FP64 and in mixed precision."""
+    from ramses2_amd.autotune import DEFAULT_SHAPE
     from ramses2_amd import _lib
     from ramses2_amd.serving import EnsembleServer, HipEngine, latency_summary, make_stream, mean_job_cells
 
@@ -244,7 +246,7 @@ def run_serving(args):
         print("bench: serving brick shapes (ms per tick of %d jobs): %s -> %s" % (
             len(cal), {"%dx%dx%d" % k: round(v, 3) for k, v in shape_ms.items()}, brick), file=sys.stderr, flush=True)
     else:
-        brick = (7, 7, 8)
+        brick = DEFAULT_SHAPE
     eng = HipEngine(precision=0, brick=brick)
     # one template context per grid size of the stream (the server's start-up)
     t0 = time.perf_counter()
@@ -329,7 +331,8 @@ def main():
         print("bench: copy probe %d B read + %d B written in %.3f ms (%.0f GB/s)"
               % (nbytes, nbytes, ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9), file=sys.stderr, flush=True)
 
-    bricks = {0: (7, 7, 8), 1: (7, 7, 8)}
+    from ramses2_amd.autotune import DEFAULT_SHAPE
+    bricks = {0: DEFAULT_SHAPE, 1: DEFAULT_SHAPE}
     fixed = {}
     for prec, opt in ((0, args.fp64_brick), (1, args.mixed_brick)):
         if opt:

@@ -17,6 +17,8 @@ import numpy as np
 from . import _lib
 from ._lib import call
 
+# the library's shape until one is set (rh_state.h)
+DEFAULT_SHAPE = (8, 6, 8)
 SHAPES = [(7, 7, 8), (7, 7, 16), (8, 6, 8), (8, 6, 16), (4, 4, 4), (4, 4, 8), (15, 7, 8), (15, 7, 16), (7, 7, 4),
           (3, 3, 8), (3, 3, 4)]
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "autotune.json")

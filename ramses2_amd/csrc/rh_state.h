@@ -102,7 +102,7 @@ struct State {
     std::map<int, LevelBricks> bricks;  // by level
     std::map<int, LevelNbor> nbors;     // by level
     // brick shape used for 3D levels
-    int brick_bx = 7, brick_by = 7, brick_bz = 8;
+    int brick_bx = 8, brick_by = 6, brick_bz = 8;
     // Godunov precision: 0 FP64, 1 FP32 reconstruction/fluxes + FP64 accumulation
     int precision = 0;
     // Courant pass of the next step fused into the Godunov kernel's epilogue:

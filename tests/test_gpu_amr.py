@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from ramses2_amd import HydroParams
+from ramses2_amd.autotune import DEFAULT_SHAPE
 from ramses2_amd.hydro import AmrHierarchy
 from tests.amr_cases import (leaf_total, oracle_mesh, oracle_step, restrict_all, smooth_state, three_level_mesh,
                              two_level_mesh)
@@ -123,6 +124,6 @@ def test_batched_levels_with_per_level_brick_shapes():
         gpu.run_device(3, use_graph=True)
     finally:
         from ramses2_amd.hydro import HipHydro
-        HipHydro.set_brick_shape(7, 7, 8)
+        HipHydro.set_brick_shape(*DEFAULT_SHAPE)
     err = rel_err_per_var(gpu.state(), om.uold)
     assert np.all(err < 1e-12), err

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from ramses2_amd import HydroParams, UniformLevel, UniformMesh, ic
+from ramses2_amd.autotune import DEFAULT_SHAPE
 from tests.helpers import oracle_params, oracle_uniform, rel_err_per_var
 
 pytestmark = pytest.mark.gpu
@@ -76,7 +77,7 @@ def test_brick_shapes_agree(brick, variant):
         g, o, _, _, _ = _one_step_3d(4, cells, p, brick=brick)
     finally:
         from ramses2_amd.hydro import HipHydro
-        HipHydro.set_brick_shape(7, 7, 8)
+        HipHydro.set_brick_shape(*DEFAULT_SHAPE)
     assert np.all(rel_err_per_var(g, o) < TOL64)
 
 
@@ -101,7 +102,7 @@ def test_fused_courant_every_shape(brick, precision):
         b = UniformLevel(mesh, cells, p)
         dtb, tb, sb, _ = b.run_device(3, precision=precision, fused_courant=1)
     finally:
-        HipHydro.set_brick_shape(7, 7, 8)
+        HipHydro.set_brick_shape(*DEFAULT_SHAPE)
     assert dta > 0 and dtb > 0 and np.isfinite(dtb) and abs(dtb - dta) < 0.05 * dta
     if precision == 0:
         assert tb == ta
