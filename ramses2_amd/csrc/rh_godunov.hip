@@ -500,7 +500,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
 
     // Column of this thread.  Row-major over the (2BX+2)(2BY+2) columns, or,
     // when the 4 BX BY interior columns fill whole waves and the halo columns
-    // fit one more (HW: the 8x6 brick, 192 + 60 threads), interior columns
+    // fit one more (HW: 8x6 bricks, 192 + 60 threads; 4x4, 64 + 36), interior columns
     // first and the halo ring last, so that the halo wave, which needs at most
     // one face per column and no update, runs one solver pass instead of three
     // and skips the update; it takes the P-phase ring instead.
