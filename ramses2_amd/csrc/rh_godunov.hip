@@ -392,12 +392,15 @@ __device__ __forceinline__ void reflux_add(double *unew, const int32_t *ghost_pa
     for (int n = 0; n < 5; n++) atomicAdd(p + n * ttd, sign * (F[n] * w));
 }
 
-// Loads of the next plane's P-phase cells issued a plane ahead, per precision.
+// Loads of the next plane's P-phase cells issued a plane ahead, per precision
+// (FP32: the per-level kernel only; it runs at two waves per SIMD with them,
+// measured 4% faster at 256^3 than at three without, while the work-list
+// kernel keeps four waves per SIMD).
 #ifndef RH_PREFETCH_F64
 #define RH_PREFETCH_F64 1
 #endif
 #ifndef RH_PREFETCH_F32
-#define RH_PREFETCH_F32 0
+#define RH_PREFETCH_F32 1
 #endif
 
 // LDS of one brick workgroup.  NS primitive planes are kept: 2 when the slopes
@@ -599,12 +602,12 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         cell_load3(r, u, rf);
         fl = r.flip;
     };
-    // prefetch a plane ahead (about 45 more VGPRs, load latency hidden): only
-    // where the registers are free without losing a wave per SIMD, i.e. the
-    // FP64 HLLC variant (183 -> 230 of 256); elsewhere the loads are issued at
-    // use, as the other variants would spill or drop to fewer waves
-    // (it also needs a thread for every ring cell)
-    constexpr bool PF = (sizeof(T) == 8 ? RH_PREFETCH_F64 : RH_PREFETCH_F32) && RS == 3 && NS == 2 &&
+    // prefetch a plane ahead (about 45 more VGPRs, load latency hidden) in the
+    // HLLC variants: the FP64 kernels, and the FP32 per-level kernel, which
+    // drops to two waves per SIMD with it and is faster for it; the FP32
+    // work-list kernel and the other variants load at use, as they would
+    // spill or lose waves (it also needs a thread for every ring cell)
+    constexpr bool PF = (sizeof(T) == 8 ? RH_PREFETCH_F64 : RH_PREFETCH_F32 && !WL) && RS == 3 && NS == 2 &&
                         (HW ? NRING <= NT - NI : NRING <= NT);
     double u_o[5], u_r[5];
     uint8_t rf_o, rf_r = 0;
