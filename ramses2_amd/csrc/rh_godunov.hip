@@ -480,17 +480,14 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     const double dtxhalf = dt * 0.5;
     const double gdt[3] = {a.g[0] * dtxhalf, a.g[1] * dtxhalf, a.g[2] * dtxhalf};
     const double inv_dx = a.inv_dx;
-    // face flux times dt/dx in FP64: (F dt)/dx in the reference's order for the
-    // FP64 kernel; F*(dt/dx) with one product for the mixed kernel.  Each face
-    // flux is scaled once and used by both cells, so the update conserves.
+    // The face fluxes stay unscaled (in LDS and in registers); the update
+    // adds dt/dx times each difference of the two faces of a dimension with
+    // one FMA, u + (dt/dx)(F(i) - F(i+1)), both cells of a face reading the
+    // same flux, so the update conserves.  The reference stores (F dt)/dx per
+    // face (hydro/umuscl.f90:135-220); that product, in its order for FP64,
+    // is formed only for the coarse reflux.
     const double dtdx64 = dt * inv_dx;
-    auto fscale = [&](T f) -> double { return sizeof(T) == 4 ? double(f) * dtdx64 : (double(f) * dt) * inv_dx; };
-    // FP64: the x and y face fluxes are scaled by the thread that solves the
-    // face, before they go to LDS, instead of by each of the two cells that
-    // read them (the same products, half as many); FP32 keeps its fluxes in
-    // FP32 in LDS and scales at the read
-    constexpr bool PRE = sizeof(T) == 8;
-    auto fread = [&](T f) -> double { return PRE ? double(f) : fscale(f); };
+    auto fscale = [&](double f) -> double { return sizeof(T) == 4 ? f * dtdx64 : (f * dt) * inv_dx; };
     const int boxmin[3] = {a.boxmin[0], a.boxmin[1], a.boxmin[2]};
     const int boxmax[3] = {a.boxmax[0], a.boxmax[1], a.boxmax[2]};
     int refl = 0;
@@ -784,7 +781,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 T f[5] = {g[0], dy ? g[2] : g[1], dy ? g[1] : g[2], g[3], g[4]};
 #pragma unroll
                 for (int n = 0; n < 5; n++) {
-                    f[n] = z ? T(0) : (PRE ? T(fscale(f[n])) : f[n]);
+                    f[n] = z ? T(0) : f[n];
                     fx[n] = f[n];
                     fy[n] = f[n];
                 }
@@ -797,7 +794,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 face_flux<RS>(0, ql, qp[0], fx, k);
                 const bool z = myref || sref[s0][pc - 1];
 #pragma unroll
-                for (int n = 0; n < 5; n++) fx[n] = z ? T(0) : (PRE ? T(fscale(fx[n])) : fx[n]);
+                for (int n = 0; n < 5; n++) fx[n] = z ? T(0) : fx[n];
             }
             if (wfy) {
                 T ql[5];
@@ -806,7 +803,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 face_flux<RS>(1, ql, qp[1], fy, k);
                 const bool z = myref || sref[s0][pc - PX];
 #pragma unroll
-                for (int n = 0; n < 5; n++) fy[n] = z ? T(0) : (PRE ? T(fscale(fy[n])) : fy[n]);
+                for (int n = 0; n < 5; n++) fy[n] = z ? T(0) : fy[n];
             }
         }
         if (interior && tt >= tlo && tt <= thi + 1) {
@@ -814,7 +811,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             face_flux<RS>(2, qmz_prev, qp[2], f, k);
             const bool z = myref || ref_prev;
 #pragma unroll
-            for (int n = 0; n < 5; n++) fz[n] = z ? 0.0 : fscale(f[n]);
+            for (int n = 0; n < 5; n++) fz[n] = z ? 0.0 : double(f[n]);
         }
         if (a.reflux) {
             // window value and ownership of a cell of the trace region
@@ -828,21 +825,23 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             if (wfx && !(myref || sref[s0][pc - 1])) {
                 const int vl = wv(cx - 1, cy, tt), vh = wv(cx, cy, tt);
 #pragma unroll
-                for (int n = 0; n < 5; n++) F[n] = fread(fx[n]);
+                for (int n = 0; n < 5; n++) F[n] = fscale(double(fx[n]));
                 if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc);
                 if (vh > a.ngridmax && owned(cx - 1, cy, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc);
             }
             if (wfy && !(myref || sref[s0][pc - PX])) {
                 const int vl = wv(cx, cy - 1, tt), vh = wv(cx, cy, tt);
 #pragma unroll
-                for (int n = 0; n < 5; n++) F[n] = fread(fy[n]);
+                for (int n = 0; n < 5; n++) F[n] = fscale(double(fy[n]));
                 if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc);
                 if (vh > a.ngridmax && owned(cx, cy - 1, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc);
             }
             if (interior && tt >= tlo && tt <= thi + 1 && !(myref || ref_prev)) {
                 const int vl = wv(cx, cy, tt - 1), vh = wv(cx, cy, tt);
-                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, fz, 8, a.racc);
-                if (vh > a.ngridmax && owned(cx, cy, tt - 1, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, fz, 8, a.racc);
+#pragma unroll
+                for (int n = 0; n < 5; n++) F[n] = fscale(fz[n]);
+                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc);
+                if (vh > a.ngridmax && owned(cx, cy, tt - 1, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc);
             }
         }
         if constexpr (!SEP) __syncthreads();  // the fluxes overwrite the face states in sf
@@ -867,11 +866,9 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 }
 #pragma unroll
                 for (int n = 0; n < 5; n++) {
-                    const double fl = fread(fb[0][n][col]);
-                    const double fr = fread(fb[0][n][col + 1]);
-                    const double gl = fread(fb[1][n][col]);
-                    const double gr = fread(fb[1][n][col + CX]);
-                    part[n] = (ub[n] + (fl - fr)) + (gl - gr);
+                    const double fl = fb[0][n][col], fr = fb[0][n][col + 1];
+                    const double gl = fb[1][n][col], gr = fb[1][n][col + CX];
+                    part[n] = __builtin_fma(dtdx64, gl - gr, __builtin_fma(dtdx64, fl - fr, ub[n]));
                 }
             }
             const int tf = tt - 1;
@@ -886,7 +883,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                     double un[5];
 #pragma unroll
                     for (int n = 0; n < 5; n++) {
-                        un[n] = part_prev[n] + (fz_prev[n] - fz[n]);
+                        un[n] = __builtin_fma(dtdx64, fz_prev[n] - fz[n], part_prev[n]);
                         po[8 * n] = un[n];
                     }
                     if (CU && a.courant && !ref_prev) {

@@ -236,9 +236,9 @@ __device__ __forceinline__ void riemann_hllc(const T ql[5], const T qr[5], T f[5
     T Pr = tmax(qr[4], rr * k.smallp_riem);
     T er = Pr * k.entho;
     T etotr = er + half * rr * (ur * ur + vr * vr + wr * wr);
-    T cfastl = sqrt_pos(tmax(tdiv(k.gamma * Pl, rl), k.smallc2));
-    T cfastr = sqrt_pos(tmax(tdiv(k.gamma * Pr, rr), k.smallc2));
-    T cmax = tmax(cfastl, cfastr);
+    // max(sqrt(a), sqrt(b)) = sqrt(max(a, b)) for a correctly rounded
+    // (monotonic) square root: one root instead of the two sound speeds
+    T cmax = sqrt_pos(tmax(tmax(tdiv(k.gamma * Pl, rl), tdiv(k.gamma * Pr, rr)), k.smallc2));
     T SL = tmin(ul, ur) - cmax;
     T SR = tmax(ul, ur) + cmax;
     T rcl = rl * (ul - SL);
