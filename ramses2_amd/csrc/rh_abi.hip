@@ -549,6 +549,7 @@ int rh_alloc_amr(int ngridmax, int ncachemax, int nvar, int twotondim, int hash_
     }
     s.h_ckey.assign((size_t)s.ngridtot * 3, 0);
     s.h_lev.assign((size_t)s.ngridtot, 0);
+    s.h_refany.clear();
     RH_HIP_CHECK(hipStreamSynchronize(s.stream));
     return 0;
 }
@@ -598,6 +599,7 @@ int rh_set_grid_device(const double *uold, const double *unew, const void *bold,
         const size_t rec = s.ndim == 3 ? 32 : 24;
         const size_t off_ckey = 8, off_ref = s.ndim == 3 ? 20 : 12, off_lev = s.ndim == 3 ? 28 : 16;
         std::vector<uint8_t> refb((size_t)ngridmax * 8, 0);
+        s.h_refany.assign((size_t)ngridmax, 0);
         const uint8_t *g = (const uint8_t *)grid;
         for (int i = 0; i < ngridmax; i++) {
             const uint8_t *r = g + rec * i;
@@ -606,6 +608,7 @@ int rh_set_grid_device(const double *uold, const double *unew, const void *bold,
             memcpy(&s.h_ckey[3 * (size_t)i], ck, sizeof(ck));
             memcpy(&s.h_lev[i], r + off_lev, sizeof(int32_t));
             memcpy(&refb[8 * (size_t)i], r + off_ref, s.ttd);
+            for (int c = 0; c < s.ttd; c++) s.h_refany[i] |= refb[8 * (size_t)i + c] != 0;
         }
         RH_HIP_CHECK(hipMemcpyAsync(s.ckey, s.h_ckey.data(), (size_t)ngridmax * 3 * sizeof(int32_t), hipMemcpyHostToDevice, s.stream));
         RH_HIP_CHECK(hipMemcpyAsync(s.lev, s.h_lev.data(), (size_t)ngridmax * sizeof(int32_t), hipMemcpyHostToDevice, s.stream));
@@ -824,6 +827,12 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
             }
         }
         lb.nbricks = (int)keys.size();
+        // refined cells in the level (unknown before a grid upload: assume some)
+        lb.any_ref = 1;
+        if (s.h_refany.size() >= (size_t)(head_idx - 1 + num_octs)) {
+            lb.any_ref = 0;
+            for (int i = head_idx - 1; i < head_idx - 1 + num_octs && !lb.any_ref; i++) lb.any_ref = s.h_refany[i];
+        }
         std::vector<int32_t> org(3 * keys.size());
         for (size_t b = 0; b < keys.size(); b++) {
             const int64_t k = keys[b];
@@ -968,6 +977,7 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         a.win = lb.win;
         a.org = lb.org;
         a.zr = lb.zr;
+        a.any_ref = lb.any_ref;
         a.nbricks = lb.nbricks;
         a.head = head_idx;
         a.noct = num_subgrids;

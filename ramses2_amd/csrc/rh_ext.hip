@@ -57,6 +57,7 @@ int fill_args3(int head_idx, int num_octs, int ilevel, double dx, double gamma, 
     a.win = lb.win;
     a.org = lb.org;
     a.zr = lb.zr;
+    a.any_ref = lb.any_ref;
     a.nbricks = lb.nbricks;
     a.head = head_idx;
     a.noct = num_octs;

@@ -35,6 +35,9 @@ struct GodArgs {
     // coarse-fine refluxing (levels above levelmin): octs > ngridmax are
     // interpolated ghosts whose parent cell is ghost_parent[2*(v-ngridmax-1)+{0,1}]
     int reflux;
+    // 0 when no cell of the level is refined: the kernel then skips the
+    // refined flags (their loads, the zeroing of refined faces)
+    int any_ref;
     int ngridmax;
     const int32_t *ghost_parent;
     // fused courant_fine of the updated state: per-brick partials

@@ -490,6 +490,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     auto fscale = [&](double f) -> double { return sizeof(T) == 4 ? f * dtdx64 : (f * dt) * inv_dx; };
     const int boxmin[3] = {a.boxmin[0], a.boxmin[1], a.boxmin[2]};
     const int boxmax[3] = {a.boxmax[0], a.boxmax[1], a.boxmax[2]};
+    const bool anyref = a.any_ref != 0;  // refined flags are all 0 when not
     int refl = 0;
 #pragma unroll
     for (int d = 0; d < 3; d++)
@@ -595,7 +596,8 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         const int v = swin[(oz * WY + oy) * WX + ox];
         const int okey[3] = {ox0 + ox - 1, oy0 + oy - 1, oz0 + oz - 1};
         const int c[3] = {px & 1, py & 1, tz & 1};
-        const CellRef r = cell_ref3(a.uold, a.refined, v, okey, boxmin, boxmax, refl, c);
+        CellRef r = cell_ref3(a.uold, a.refined, v, okey, boxmin, boxmax, refl, c);
+        if (!anyref) r.rf = nullptr;
         cell_load3(r, u, rf);
         fl = r.flip;
     };
@@ -751,7 +753,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         }
         T qp[3][5], qm[3][5];
         trace3(q0, dq, qp, qm, k, inv_r0);
-        const uint8_t myref = sref[s0][pc];
+        const uint8_t myref = anyref ? sref[s0][pc] : 0;
 #pragma unroll
         for (int n = 0; n < 5; n++) {
             sf[0][n][col] = qm[0][n];
@@ -777,11 +779,14 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 a5[0] = ql[0], a5[1] = dy ? ql[2] : ql[1], a5[2] = dy ? ql[1] : ql[2], a5[3] = ql[3], a5[4] = ql[4];
                 b5[0] = qr[0], b5[1] = dy ? qr[2] : qr[1], b5[2] = dy ? qr[1] : qr[2], b5[3] = qr[3], b5[4] = qr[4];
                 riemann_hllc(a5, b5, g, k);
-                const bool z = myref || sref[s0][dy ? pc - PX : pc - 1];
                 T f[5] = {g[0], dy ? g[2] : g[1], dy ? g[1] : g[2], g[3], g[4]};
+                if (anyref) {
+                    const bool z = myref || sref[s0][dy ? pc - PX : pc - 1];
+#pragma unroll
+                    for (int n = 0; n < 5; n++) f[n] = z ? T(0) : f[n];
+                }
 #pragma unroll
                 for (int n = 0; n < 5; n++) {
-                    f[n] = z ? T(0) : f[n];
                     fx[n] = f[n];
                     fy[n] = f[n];
                 }
@@ -792,26 +797,34 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
 #pragma unroll
                 for (int n = 0; n < 5; n++) ql[n] = sf[0][n][col - 1];
                 face_flux<RS>(0, ql, qp[0], fx, k);
-                const bool z = myref || sref[s0][pc - 1];
+                if (anyref) {
+                    const bool z = myref || sref[s0][pc - 1];
 #pragma unroll
-                for (int n = 0; n < 5; n++) fx[n] = z ? T(0) : fx[n];
+                    for (int n = 0; n < 5; n++) fx[n] = z ? T(0) : fx[n];
+                }
             }
             if (wfy) {
                 T ql[5];
 #pragma unroll
                 for (int n = 0; n < 5; n++) ql[n] = sf[1][n][col - CX];
                 face_flux<RS>(1, ql, qp[1], fy, k);
-                const bool z = myref || sref[s0][pc - PX];
+                if (anyref) {
+                    const bool z = myref || sref[s0][pc - PX];
 #pragma unroll
-                for (int n = 0; n < 5; n++) fy[n] = z ? T(0) : fy[n];
+                    for (int n = 0; n < 5; n++) fy[n] = z ? T(0) : fy[n];
+                }
             }
         }
         if (interior && tt >= tlo && tt <= thi + 1) {
             T f[5];
             face_flux<RS>(2, qmz_prev, qp[2], f, k);
-            const bool z = myref || ref_prev;
 #pragma unroll
-            for (int n = 0; n < 5; n++) fz[n] = z ? 0.0 : double(f[n]);
+            for (int n = 0; n < 5; n++) fz[n] = double(f[n]);
+            if (anyref) {
+                const bool z = myref || ref_prev;
+#pragma unroll
+                for (int n = 0; n < 5; n++) fz[n] = z ? 0.0 : fz[n];
+            }
         }
         if (a.reflux) {
             // window value and ownership of a cell of the trace region

@@ -36,6 +36,7 @@ struct LevelBricks {
     int32_t *win = nullptr;   // [nbricks][bz+2][by+2][bx+2]
     int32_t *org = nullptr;   // [nbricks][3] oct key of the brick's first oct
     int32_t *zr = nullptr;    // [nbricks] z layers holding octs of the level: zlo | zhi << 16
+    int any_ref = 1;          // some cell of the level's octs is refined
     int boxmin[3] = {0, 0, 0};
     int boxmax[3] = {0, 0, 0};
 };
@@ -99,6 +100,7 @@ struct State {
     int btype[3][2] = {{2, 2}, {2, 2}, {2, 2}};
     // host copy of the oct keys (brick enumeration)
     std::vector<int32_t> h_ckey, h_lev;
+    std::vector<uint8_t> h_refany;  // per oct: some cell refined (from the last grid upload)
     std::map<int, LevelBricks> bricks;  // by level
     std::map<int, LevelNbor> nbors;     // by level
     // brick shape used for 3D levels
