@@ -70,7 +70,6 @@ hipError_t enqueue_amr_step(State &s, int parity, int precision)
     hipError_t e;
     for (int l : m.levels)
         if (l > m.levelmin && (e = refresh_level_ghosts(s, l, ucur)) != hipSuccess) return e;
-    if ((e = launch_zero(m.racc, 5l * std::max(s.nghost_total, 1), s.stream)) != hipSuccess) return e;
     for (const auto &gr : m.groups) {
         WorkList w;
         w.descs = m.d_descs + parity * m.ndesc;
@@ -166,6 +165,8 @@ int rh_amr_prepare(int levelmin, int levelmax, const int32_t *head, const int32_
     m.d_descs = nullptr;
     m.d_items = nullptr;
     RH_HIP_CHECK(dmalloc((void **)&m.racc, 5 * sizeof(double) * (size_t)std::max(s.nghost_total, 1)));
+    // zeroed once: each step's fold leaves it zeroed again
+    RH_HIP_CHECK(launch_zero(m.racc, 5l * std::max(s.nghost_total, 1), s.stream));
     for (auto &a : descs) a.racc = m.racc;
     m.ndesc = (int)descs.size();
     std::vector<GodArgs> both(2 * descs.size());

@@ -114,7 +114,7 @@ hipError_t launch_courant_ml(const double *uold, const uint8_t *refined, const i
 // racc[0 .. 5*n) = 0 (a kernel: safe inside captured graphs)
 hipError_t launch_zero(double *p, long n, hipStream_t s);
 // unew(parent cell of ghost g) += racc[5*g .. 5*g+4] for the ghosts g in [0, nghost)
-hipError_t launch_reflux_fold(double *unew, const double *racc, const int32_t *ghost_parent, int nghost, int ttd,
+hipError_t launch_reflux_fold(double *unew, double *racc, const int32_t *ghost_parent, int nghost, int ttd,
                               hipStream_t s);
 hipError_t launch_courant_reduce_batch(const CourantJob *jobs, int njobs, hipStream_t s);
 hipError_t launch_courant_reduce(const double *partials, int nparts, double dtmax, double *out, double *dt_inout,

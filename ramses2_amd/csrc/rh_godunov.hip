@@ -1208,17 +1208,30 @@ __global__ void k_zero(double *p, long n)
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
 }
 
-// Coarse-fine reflux fold: each ghost oct stands for one parent cell.
-__global__ void k_reflux_fold(double *unew, const double *__restrict__ racc, const int32_t *__restrict__ ghost_parent,
+// Coarse-fine reflux fold: each ghost oct stands for one parent cell.  The
+// accumulator is left zeroed for the next step (it starts zeroed); ghosts
+// that received no flux leave their parent cell untouched, as the direct
+// reflux of the per-level calls does.
+__global__ void k_reflux_fold(double *unew, double *__restrict__ racc, const int32_t *__restrict__ ghost_parent,
                               int nghost, int ttd)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nghost) return;
+    double f[5];
+    bool any = false;
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+        f[n] = racc[5 * (size_t)g + n];
+        any = any || f[n] != 0.0;
+    }
+    if (!any) return;
+#pragma unroll
+    for (int n = 0; n < 5; n++) racc[5 * (size_t)g + n] = 0.0;
     const int igrid = ghost_parent[2 * g], icell = ghost_parent[2 * g + 1];
     if (igrid <= 0) return;
     double *p = unew + (size_t)(igrid - 1) * 5 * ttd + (icell - 1);
 #pragma unroll
-    for (int n = 0; n < 5; n++) p[n * ttd] = p[n * ttd] + racc[5 * (size_t)g + n];
+    for (int n = 0; n < 5; n++) p[n * ttd] = p[n * ttd] + f[n];
 }
 
 // Fixed-order reduction of the per-block partials by one 256-thread block:
@@ -1387,7 +1400,13 @@ __global__ void __launch_bounds__(128) k_refresh_ghosts(const GhostArgs a)
 #pragma unroll
     for (int j = 0; j < nslot; j++) {
         const int code = a.src[(g * 7 + j) * 2], ind = a.src[(g * 7 + j) * 2 + 1];
-        const int okey[3] = {a.srckey[(g * 7 + j) * 3], a.srckey[(g * 7 + j) * 3 + 1], a.srckey[(g * 7 + j) * 3 + 2]};
+        // the source's oct key locates boundary ghosts only (code < 0)
+        int okey[3] = {0, 0, 0};
+        if (code < 0) {
+            okey[0] = a.srckey[(g * 7 + j) * 3];
+            okey[1] = a.srckey[(g * 7 + j) * 3 + 1];
+            okey[2] = a.srckey[(g * 7 + j) * 3 + 2];
+        }
         int c[3] = {(ind - 1) & 1, ((ind - 1) >> 1) & 1, ((ind - 1) >> 2) & 1};
         uint8_t rf;
         if constexpr (ND == 3) {
@@ -1644,7 +1663,7 @@ hipError_t launch_zero(double *p, long n, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_reflux_fold(double *unew, const double *racc, const int32_t *ghost_parent, int nghost, int ttd,
+hipError_t launch_reflux_fold(double *unew, double *racc, const int32_t *ghost_parent, int nghost, int ttd,
                               hipStream_t s)
 {
     if (nghost <= 0) return hipSuccess;
