@@ -478,6 +478,7 @@ int rh_finalize(void)
     dfree(s.ghost_parent);
     dfree(s.ghost_src);
     dfree(s.ghost_srckey);
+    dfree(s.ghost_order);
     dfree(s.ghost_key);
     dfree(s.d_counter);
     dfree(s.red_part);
@@ -539,10 +540,12 @@ int rh_alloc_amr(int ngridmax, int ncachemax, int nvar, int twotondim, int hash_
         dfree(s.ghost_parent);
         dfree(s.ghost_src);
         dfree(s.ghost_srckey);
+        dfree(s.ghost_order);
         dfree(s.ghost_key);
         if (dmalloc((void **)&s.ghost_parent, ng * 2 * sizeof(int32_t)) != hipSuccess ||
             dmalloc((void **)&s.ghost_src, ng * 14 * sizeof(int32_t)) != hipSuccess ||
             dmalloc((void **)&s.ghost_srckey, ng * 21 * sizeof(int32_t)) != hipSuccess ||
+            dmalloc((void **)&s.ghost_order, ng * sizeof(int32_t)) != hipSuccess ||
             dmalloc((void **)&s.ghost_key, ng * 3 * sizeof(int32_t)) != hipSuccess)
             return fail(RH_ERR_NOMEM, "device allocation failed for the ghost-oct tables");
         s.nghost_total = 0;
@@ -755,6 +758,25 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
                 ln.ghost_first = s.ngridmax + 1 + base;
                 ln.nghost = nnew;
                 s.nghost_total += nnew;
+                {
+                    // the level's ghosts in parent-cell order: the interpolation
+                    // and the reflux fold walk them in this order, so that the
+                    // ghosts of one wave (and of one XCD) share parent octs
+                    std::vector<int32_t> par((size_t)nnew * 2), ord((size_t)nnew);
+                    RH_HIP_CHECK(hipMemcpyAsync(par.data(), s.ghost_parent + 2 * (size_t)base,
+                                                par.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream));
+                    RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+                    for (int g = 0; g < nnew; g++) ord[g] = g;
+                    std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
+                        const int64_t kx = (int64_t)par[2 * x] * 8 + par[2 * x + 1];
+                        const int64_t ky = (int64_t)par[2 * y] * 8 + par[2 * y + 1];
+                        return kx < ky;
+                    });
+                    for (int g = 0; g < nnew; g++) ord[g] += base;
+                    RH_HIP_CHECK(hipMemcpyAsync(s.ghost_order + base, ord.data(), ord.size() * sizeof(int32_t),
+                                                hipMemcpyHostToDevice, s.stream));
+                    RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+                }
                 for (int g = 0; g < nnew; g++) s.h_lev[s.ngridmax + base + g] = ilevel;
                 if (hc[2] & 1) {
                     s.nbors[ilevel] = ln;
@@ -949,6 +971,8 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         const int gi0 = lnb.ghost_first - s.ngridmax - 1;
         ga.src = s.ghost_src + (size_t)gi0 * 14;
         ga.srckey = s.ghost_srckey + (size_t)gi0 * 21;
+        ga.order = s.ghost_order + gi0;
+        ga.gi0 = gi0;
         ga.first = lnb.ghost_first;
         ga.nghost = lnb.nghost;
         ga.ndim = s.ndim;

@@ -46,6 +46,8 @@ hipError_t refresh_level_ghosts(State &s, int ilevel, double *u)
     const int gi0 = lnb.ghost_first - s.ngridmax - 1;
     ga.src = s.ghost_src + (size_t)gi0 * 14;
     ga.srckey = s.ghost_srckey + (size_t)gi0 * 21;
+    ga.order = s.ghost_order + gi0;
+    ga.gi0 = gi0;
     ga.first = lnb.ghost_first;
     ga.nghost = lnb.nghost;
     ga.ndim = 3;
@@ -78,7 +80,7 @@ hipError_t enqueue_amr_step(State &s, int parity, int precision)
         if ((e = launch_godunov3d_wl(w, gr.bx, gr.by, gr.bz, m.slope, m.riemann, precision, s.stream)) != hipSuccess)
             return e;
     }
-    if ((e = launch_reflux_fold(unext, m.racc, s.ghost_parent, s.nghost_total, 8, s.stream)) != hipSuccess) return e;
+    if ((e = launch_reflux_fold(unext, m.racc, s.ghost_parent, s.ghost_order, s.nghost_total, 8, s.stream)) != hipSuccess) return e;
     for (int i = (int)m.levels.size() - 1; i >= 1; i--) {
         const int lf = m.levels[i];
         if (m.levels[i - 1] != lf - 1) continue;

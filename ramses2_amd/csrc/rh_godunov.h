@@ -114,8 +114,8 @@ hipError_t launch_courant_ml(const double *uold, const uint8_t *refined, const i
 // racc[0 .. 5*n) = 0 (a kernel: safe inside captured graphs)
 hipError_t launch_zero(double *p, long n, hipStream_t s);
 // unew(parent cell of ghost g) += racc[5*g .. 5*g+4] for the ghosts g in [0, nghost)
-hipError_t launch_reflux_fold(double *unew, double *racc, const int32_t *ghost_parent, int nghost, int ttd,
-                              hipStream_t s);
+hipError_t launch_reflux_fold(double *unew, double *racc, const int32_t *ghost_parent, const int32_t *order,
+                              int nghost, int ttd, hipStream_t s);
 hipError_t launch_courant_reduce_batch(const CourantJob *jobs, int njobs, hipStream_t s);
 hipError_t launch_courant_reduce(const double *partials, int nparts, double dtmax, double *out, double *dt_inout,
                                  double *t_acc, hipStream_t s);
@@ -133,6 +133,8 @@ struct GhostArgs {
     const uint8_t *refined;
     const int32_t *src;      // [nghost][7][2]
     const int32_t *srckey;   // [nghost][7][3]
+    const int32_t *order;    // [nghost] ghost indices (0-based, all levels) in parent-cell order
+    int gi0;                 // ghost index of the level's first ghost
     int first;               // oct index (1-based) of the first ghost of the level
     int nghost;
     int ndim;

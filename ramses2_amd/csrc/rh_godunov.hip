@@ -1219,6 +1219,14 @@ __global__ void __launch_bounds__(kCourantThreads) k_courant_ml(const double *__
     }
 }
 
+// Block b of a grid of nb blocks renumbered so that the blocks the hardware
+// sends to one XCD (b mod 8) take one contiguous range of [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    const int x = b & 7, l = b >> 3, q = nb >> 3, r = nb & 7;
+    return x * q + (x < r ? x : r) + l;
+}
+
 __global__ void k_zero(double *p, long n)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
@@ -1229,10 +1237,12 @@ __global__ void k_zero(double *p, long n)
 // that received no flux leave their parent cell untouched, as the direct
 // reflux of the per-level calls does.
 __global__ void k_reflux_fold(double *unew, double *__restrict__ racc, const int32_t *__restrict__ ghost_parent,
-                              int nghost, int ttd)
+                              const int32_t *__restrict__ order, int nghost, int ttd)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nghost) return;
+    // in parent-cell order (see k_refresh_ghosts)
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= nghost) return;
+    const int g = order[t];
     double f[5];
     bool any = false;
 #pragma unroll
@@ -1408,8 +1418,11 @@ __device__ __forceinline__ void interpol_hydro_dev(double u1[7][5], double u2[8]
 template <int ND>
 __global__ void __launch_bounds__(128) k_refresh_ghosts(const GhostArgs a)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.nghost) return;
+    // ghosts in parent-cell order, a contiguous range of that order per XCD,
+    // so that the parent octs a wave gathers meet in one L2
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= a.nghost) return;
+    const int g = a.order[t] - a.gi0;
     constexpr int ttd = 1 << ND, nslot = 2 * ND + 1;
     const int stride = ttd * 5;
     double u1[7][5], u2[8][5];
@@ -1679,12 +1692,12 @@ hipError_t launch_zero(double *p, long n, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_reflux_fold(double *unew, double *racc, const int32_t *ghost_parent, int nghost, int ttd,
-                              hipStream_t s)
+hipError_t launch_reflux_fold(double *unew, double *racc, const int32_t *ghost_parent, const int32_t *order,
+                              int nghost, int ttd, hipStream_t s)
 {
     if (nghost <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reflux_fold, dim3((nghost + 255) / 256), dim3(256), 0, s, unew, racc, ghost_parent, nghost,
-                       ttd);
+    hipLaunchKernelGGL(k_reflux_fold, dim3((nghost + 255) / 256), dim3(256), 0, s, unew, racc, ghost_parent, order,
+                       nghost, ttd);
     return hipGetLastError();
 }
 

@@ -189,6 +189,7 @@ int rh_ctx_clone_mesh(int src_id)
     }
     s.h_ckey = src->h_ckey;
     s.h_lev = src->h_lev;
+    s.h_refany = src->h_refany;
     s.brick_bx = src->brick_bx;
     s.brick_by = src->brick_by;
     s.brick_bz = src->brick_bz;
@@ -215,6 +216,7 @@ int rh_ctx_clone_mesh(int src_id)
     RH_HIP_CHECK(dup((void **)&s.ghost_parent, src->ghost_parent, ng * 2 * sizeof(int32_t)));
     RH_HIP_CHECK(dup((void **)&s.ghost_src, src->ghost_src, ng * 14 * sizeof(int32_t)));
     RH_HIP_CHECK(dup((void **)&s.ghost_srckey, src->ghost_srckey, ng * 21 * sizeof(int32_t)));
+    RH_HIP_CHECK(dup((void **)&s.ghost_order, src->ghost_order, ng * sizeof(int32_t)));
     RH_HIP_CHECK(dup((void **)&s.ghost_key, src->ghost_key, ng * 3 * sizeof(int32_t)));
     const int nslot = s.ndim == 3 ? 27 : 3;
     for (const auto &kv : src->nbors) {
@@ -264,7 +266,7 @@ int rh_estimate_uniform_bytes(int n_octs_per_dim, int ncachemax, int hash_size, 
     b += 2 * ntot * 40 * 8;                                   // uold, unew
     b += ntot * (12 + 4 + 8);                                 // ckey, lev, refined
     b += cap * (8 + 4);                                       // hash keys, values
-    b += (int64_t)std::max(ncachemax, 1) * (2 + 14 + 21 + 3) * 4;  // ghost-oct tables
+    b += (int64_t)std::max(ncachemax, 1) * (2 + 14 + 21 + 1 + 3) * 4;  // ghost-oct tables
     b += 8 * 8 + 2 * 8 + 4 * 4 + 2 * 8;                       // reduction / dt / counters
     b += std::max<int64_t>(4096, nb) * 4 * 8;                 // Courant partials
     b += noct * 27 * 4 + noct * 2 * 4;                        // neighbour and father tables

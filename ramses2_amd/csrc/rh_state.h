@@ -120,6 +120,7 @@ struct State {
     int32_t *ghost_parent = nullptr;  // [ncachemax][2]
     int32_t *ghost_src = nullptr;     // [ncachemax][7][2]
     int32_t *ghost_srckey = nullptr;  // [ncachemax][7][3]
+    int32_t *ghost_order = nullptr;   // [ncachemax] ghost indices, each level's range in parent-cell order
     int32_t *ghost_key = nullptr;     // [ncachemax][3]
     int32_t *d_counter = nullptr;     // device scratch counters [4]
     int interpol_var = 0, interpol_type = 1;
