@@ -7,13 +7,16 @@
 //   P: gather plane t of the (2BX+4) x (2BY+4) primitive region from uold
 //      through the brick window (hash-resolved oct indices, boundary ghosts
 //      synthesised on the fly), convert to primitives in FP64, keep a ring of
-//      three planes in LDS;
+//      two planes in LDS (three for the 27-point slope), the next plane's
+//      values already in flight (prefetched a plane ahead);
 //   T: slopes (uslope) and Hancock predictor (trace3d) of plane t-1, once per
 //      cell; left states of x/y faces go to LDS, z states stay in registers;
 //   F: each column solves the Riemann problem on its own low x, y and z face
 //      (cmpflxm), so every face is solved once; refined faces get zero flux;
-//   U: conservative update, accumulated as ((u + dFx) + dFy) + dFz in the
-//      order of godfine1 (hydro/godunov_fine.f90:737-816), written to unew.
+//   U: conservative update u + (dt/dx)(F(i) - F(i+1)), dimension by dimension
+//      in the order of godfine1 (hydro/godunov_fine.f90:737-816), written to
+//      unew; the courant_fine sums and dt of the new state are reduced from
+//      registers as it is stored.
 // No MFMA: the work is divides, square roots and compares, not a contraction.
 #include <hip/hip_ext.h>
 
