@@ -94,6 +94,9 @@ def lib():
                                               ctypes.c_double, ctypes.POINTER(Params),
                                               _ip, ctypes.c_int]
         L.ora_godunov_uniform_omp.restype = ctypes.c_int
+        L.ora_probe_get_grid.argtypes = [ctypes.POINTER(Mesh), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int64), _dp, _u8p]
+        L.ora_probe_get_grid.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -231,9 +234,133 @@ class OracleMesh:
     def set_uold(self, ilevel):
         lib().ora_set_uold(ctypes.byref(self.m), ilevel)
 
+    def probe_get_grid(self, lev, key):
+        """get_grid of one key: (code, u[nvar, ttd], refined[ttd]); code > 0 an
+        oct of the mesh, -1 a boundary ghost, 0 missing."""
+        ttd = 2 ** self.ndim
+        k = (ctypes.c_int64 * 3)(*([int(x) for x in key] + [0] * (3 - len(key))))
+        u = np.zeros((NVAR, ttd))
+        r = np.zeros(ttd, dtype=np.uint8)
+        c = lib().ora_probe_get_grid(ctypes.byref(self.m), lev, k, _ptr(u, _dp), _ptr(r, _u8p))
+        return c, u, r
+
     def godunov_uniform_omp(self, ilevel, dt, p, octmap, nthreads):
         octmap = np.ascontiguousarray(octmap, dtype=np.int32)
         rc = lib().ora_godunov_uniform_omp(ctypes.byref(self.m), ilevel, dt, ctypes.byref(p),
                                            _ptr(octmap, _ip), nthreads)
         if rc != 0:
             raise RuntimeError("ora_godunov_uniform_omp failed with code %d" % rc)
+
+
+# --------------------------------------------------------------------------
+# The reference's own level routines (oracle/_ref/libramses_reflev{1,3}d.so)
+# --------------------------------------------------------------------------
+_reflev = {}
+
+
+def reflev_lib(ndim):
+    """The flang-built reference level routines for ``ndim`` (None if not built)."""
+    if ndim in _reflev:
+        return _reflev[ndim]
+    path = os.path.join(HERE, "_ref", "libramses_reflev%dd.so" % ndim)
+    if not os.path.exists(path):
+        return None
+    L = ctypes.CDLL(path, mode=os.RTLD_LAZY)
+    d, i = ctypes.c_double, ctypes.c_int
+    i3 = ctypes.c_int * 3
+    L.reflev_ndim.restype = i
+    L.reflev_setup.argtypes = [i, _ip, _ip, ctypes.POINTER(ctypes.c_int8), _dp, _dp, i, i, _ip, _ip, d,
+                               i3, ctypes.c_int * 6, i, i3, i3, i, i, ctypes.c_double * 3,
+                               d, d, d, d, i, i, d]
+    L.reflev_godunov.argtypes = [i, d]
+    L.reflev_courant.argtypes = [i, _dp]
+    L.reflev_upload.argtypes = [i]
+    L.reflev_set_unew.argtypes = [i]
+    L.reflev_set_uold.argtypes = [i]
+    L.reflev_get.argtypes = [i, _dp, _dp]
+    L.reflev_get_grid.argtypes = [i, i3, _dp, ctypes.POINTER(ctypes.c_int8)]
+    L.reflev_get_grid.restype = i
+    _reflev[ndim] = L
+    return L
+
+
+class RefLevel:
+    """The mesh of an OracleMesh loaded into the reference's own types; its
+    methods run the reference's godunov_fine / courant_fine / upload_fine /
+    set_unew / set_uold (one instance at a time per ndim: the library keeps
+    the mesh in module variables).
+
+    godunov_fine and upload_fine close the reference's software cache on
+    exit, which destages every boundary ghost oct through destage()
+    (amr/task_manager.f90:974-1069).  That file's module closure takes in the
+    whole program, including lightcone/lightcone.f90, which flang rejects, so
+    it is not in the library: both are refused on meshes with a non-periodic
+    dimension, and after probe_get_grid (whose ghosts stay in the cache)."""
+
+    def __init__(self, om, p):
+        L = reflev_lib(om.ndim)
+        if L is None:
+            raise RuntimeError("reference level library not built (make -C oracle reflev)")
+        self.L, self.ngrid = L, om.m.ngrid
+        m = om.m
+        self.ndim = om.ndim
+        self.periodic = all(m.periodic[d] for d in range(om.ndim))
+        self.probed = False
+        self._ckey = np.ascontiguousarray(om.ckey, dtype=np.int32)
+        self._ref = np.ascontiguousarray(om.refined.astype(np.int8))
+        self._uold = np.ascontiguousarray(om.uold, dtype=np.float64)
+        self._unew = np.ascontiguousarray(om.unew, dtype=np.float64)
+        i3 = ctypes.c_int * 3
+        per = i3(*[m.periodic[k] for k in range(3)])
+        bt = (ctypes.c_int * 6)(*[m.bound_type[k][s] for k in range(3) for s in range(2)])
+        bmin = i3(*[m.box_ckey_min[k] for k in range(3)])
+        bmax = i3(*[m.box_ckey_max[k] for k in range(3)])
+        g = (ctypes.c_double * 3)(*[m.constant_gravity[k] for k in range(3)])
+        L.reflev_setup(m.ngrid, _ptr(self._ckey, _ip), _ptr(om.lev, _ip),
+                       self._ref.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                       _ptr(self._uold, _dp), _ptr(self._unew, _dp), m.levelmin, m.nlevelmax,
+                       _ptr(om.head, _ip), _ptr(om.noct, _ip), m.boxlen, per, bt, m.box_lev, bmin, bmax,
+                       m.interpol_var, m.interpol_type, g, p.gamma, p.smallr, p.smallc, p.courant_factor,
+                       p.slope_type, p.riemann, p.difmag)
+
+    def _cache_ok(self):
+        if not self.periodic or self.probed:
+            raise RuntimeError("closing the reference's cache needs destage (amr/task_manager.f90)")
+
+    def godunov_fine(self, ilevel, dt):
+        self._cache_ok()
+        self.L.reflev_godunov(ilevel, dt)
+
+    def courant_fine(self, ilevel):
+        out = np.zeros(5)
+        self.L.reflev_courant(ilevel, _ptr(out, _dp))
+        return out
+
+    def upload_fine(self, ilevel):
+        self._cache_ok()
+        self.L.reflev_upload(ilevel)
+
+    def probe_get_grid(self, lev, key):
+        """The reference's get_grid of one key with godunov_fine's boundary
+        initialiser: (child, u[nvar, ttd], refined[ttd]); child > ngrid is a
+        boundary ghost oct built in the cache, 0 missing."""
+        ttd = 2 ** self.ndim
+        k = (ctypes.c_int * 3)(*([int(x) for x in key] + [0] * (3 - len(key))))
+        u = np.zeros((NVAR, ttd))
+        r = np.zeros(ttd, dtype=np.int8)
+        self.probed = True
+        c = self.L.reflev_get_grid(lev, k, _ptr(u, _dp), r.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)))
+        return c, u, r.astype(np.uint8)
+
+    def set_unew(self, ilevel):
+        self.L.reflev_set_unew(ilevel)
+
+    def set_uold(self, ilevel):
+        self.L.reflev_set_uold(ilevel)
+
+    def state(self):
+        """(uold, unew) of the real octs."""
+        uo = np.zeros((self.ngrid, NVAR, self._uold.shape[2]))
+        un = np.zeros_like(uo)
+        self.L.reflev_get(self.ngrid, _ptr(uo, _dp), _ptr(un, _dp))
+        return uo, un

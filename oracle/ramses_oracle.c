@@ -1097,6 +1097,28 @@ void ora_set_uold(const ora_mesh_t *m, int ilevel)
 }
 
 /* upload_fine, hydro/upload.f90:74-283 */
+/* get_grid of one key for tests: > 0 an oct of the mesh, -1 a boundary ghost
+ * (its state and refined map returned), 0 missing.  u[nvar][ttd], refined[ttd]. */
+int ora_probe_get_grid(const ora_mesh_t *m, int lev, const int64_t *key, double *u, uint8_t *refined)
+{
+    omap_t h;
+    omap_build(&h, m);
+    lookup_t L = {m, &h, 1 << m->ndim};
+    int64_t k[3] = {key[0], key[1], key[2]};
+    ghost_t g;
+    const int ttd = L.ttd, nvar = m->nvar;
+    int c = get_grid(&L, lev, k, &g);
+    for (int ivar = 1; ivar <= nvar; ivar++)
+        for (int ind = 1; ind <= ttd; ind++)
+            u[(ivar - 1) * ttd + (ind - 1)] = c > 0    ? UOCT(m->uold, nvar, c, ivar, ind, ttd)
+                                              : c < 0 ? g.u[(ivar - 1) * ttd + (ind - 1)]
+                                                      : 0.0;
+    for (int ind = 0; ind < ttd; ind++)
+        refined[ind] = c > 0 ? m->refined[(size_t)(c - 1) * 8 + ind] : c < 0 ? g.refined[ind] : 0;
+    omap_free(&h);
+    return c;
+}
+
 int ora_upload_fine(const ora_mesh_t *m, int ilevel, double smallr)
 {
     int head, noct, fhead, fnoct;

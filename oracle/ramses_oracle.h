@@ -101,6 +101,12 @@ void ora_set_uold(const ora_mesh_t *m, int ilevel);
  * ilevel, then average each oct of level ilevel+1 into its parent cell. */
 int ora_upload_fine(const ora_mesh_t *m, int ilevel, double smallr);
 
+/* get_grid (amr/nbors_utils.f90:298-667) of one key (lev, key[0..ndim-1]),
+ * for tests: returns the oct index (> 0), -1 for a boundary ghost oct built
+ * as init_bound_refine does (amr/boundaries.f90:39-523), or 0 when missing;
+ * u[nvar][twotondim] and refined[twotondim] receive that oct's contents.    */
+int ora_probe_get_grid(const ora_mesh_t *m, int lev, const int64_t *key, double *u, uint8_t *refined);
+
 /* Uniform-level fast path for the CPU baseline: same arithmetic as
  * ora_godunov_fine on a periodic uniform level whose octs are addressed
  * through a dense (ix,iy,iz) -> oct map, run with OpenMP over octs.          */
