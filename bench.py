@@ -8,12 +8,16 @@ Courant dt reduction (courant_fine) and the Godunov update (godunov_fine with
 set_unew/set_uold folded in), all on the device.  The state is resident in HBM
 before the timed region starts.
 
-Prints one JSON line: value = FP64 cell-updates/s of the whole step; the mixed
-precision (FP32 reconstruction/fluxes, FP64 accumulation) rate and its L1
-error against the FP64 run sit beside it; roofline = the Godunov kernel's
-algorithmic HBM bytes (80 B per cell-update) over its mean launch time measured
-with HIP events inside the timed steps; cpu_baseline = the CPU oracle on the
-host cores on a bounded sample.
+Prints one JSON line: value = FP64 cell-updates/s of the whole step, the K
+timed steps replayed as the captured hipGraph of a step pair; the mixed
+precision (FP32 reconstruction/fluxes, FP64 accumulation) rate sits beside it,
+with a 100-step precision study (L1 per conserved variable against FP64, mass
+and energy drift); roofline = the Godunov kernel's algorithmic HBM bytes (80 B
+per cell-update) over its mean launch time, measured with HIP events on the
+library stream in a separate eager pass, with the kernel's VALU-issue ceiling
+from the committed counters beside it; cpu_baseline = the CPU oracle on the
+host cores on a bounded sample (all threads, and one).  Further legs: uniform
+Sedov 128^3 GPU vs CPU, AMR Sedov levels 6-10, ensemble serving.
 """
 import argparse
 import json
@@ -27,6 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+SIMDS = 1024          # 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9      # peak engine clock
 BYTES_PER_CELL = 80.0  # read uold 5x8 B + write unew 5x8 B
 
 
@@ -41,7 +47,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--graph", action="store_true", help="replay steps as a hipGraph (no per-kernel events)")
+    ap.add_argument("--eager", action="store_true",
+                    help="time the steps launch by launch instead of replaying the captured step-pair hipGraph")
+    ap.add_argument("--sedov-steps", type=int, default=200, help="steps of the uniform 128^3 Sedov leg")
+    ap.add_argument("--no-sedov", action="store_true")
+    ap.add_argument("--study-steps", type=int, default=100, help="steps of the FP64-vs-mixed precision study")
     ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs of the ensemble serving stream")
     ap.add_argument("--serve-load", type=float, default=0.8,
                     help="offered load as a fraction of the measured FP64 batched capacity")
@@ -72,21 +82,17 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(args, p):
-    """The CPU oracle (C restatement of godunov_fine/courant_fine, OpenMP over
-    octs) on a bounded sample: the same problem at cpu_sample^3 cells."""
+def cpu_rate(mesh, cells, p, threads, seconds):
+    """Full steps (courant_fine + set_unew + godunov_fine + set_uold) of the CPU
+    oracle (C restatement, OpenMP over octs) on `threads` host threads for
+    about `seconds`: cell-updates/s."""
     from oracle import oracle as O
-    from ramses2_amd import UniformMesh, ic
     from tests.helpers import oracle_params, oracle_uniform
 
-    n = args.cpu_sample
-    level = int(np.log2(n))
-    mesh = UniformMesh(3, level)
-    cells = ic.turbulence_3d(n, seed=args.seed)
+    level = mesh.level
     om = oracle_uniform(mesh, mesh.from_cells(cells))
     po = oracle_params(p)
     octmap = (mesh.octmap + 1).astype(np.int32)
-    threads = len(os.sched_getaffinity(0))
     O.lib()
 
     def one_step():
@@ -98,16 +104,52 @@ def cpu_baseline(args, p):
     t0 = time.perf_counter()
     one_step()
     t1 = time.perf_counter()
-    nsteps = max(1, min(200, int(args.cpu_seconds / max(t1 - t0, 1e-6))))
+    nsteps = max(1, min(200, int(seconds / max(t1 - t0, 1e-6))))
     t0 = time.perf_counter()
     for _ in range(nsteps):
         one_step()
     el = time.perf_counter() - t0
-    rate = nsteps * mesh.noct * 8 / el
-    return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": "CPU oracle (oracle/ramses_oracle.c, OpenMP over octs) on the same turbulence problem "
-                      "at %d^3 cells, %d full steps (courant_fine + godunov_fine), %.1f s; host CPU: %s"
-                      % (n, nsteps, el, cpu_model())}
+    return {"cell_updates_per_s": nsteps * mesh.noct * 8 / el, "threads": threads, "steps": nsteps, "seconds": el}
+
+
+def cpu_baseline(args, p):
+    """The CPU oracle on a bounded sample of the headline problem (the same
+    turbulence at cpu_sample^3 cells), on every host thread and on one."""
+    from ramses2_amd import UniformMesh, ic
+
+    n = args.cpu_sample
+    mesh = UniformMesh(3, int(np.log2(n)))
+    cells = ic.turbulence_3d(n, seed=args.seed)
+    threads = len(os.sched_getaffinity(0))
+    allt = cpu_rate(mesh, cells, p, threads, args.cpu_seconds)
+    one = cpu_rate(mesh, cells, p, 1, args.cpu_seconds / 3)
+    return {"value": allt["cell_updates_per_s"], "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "one_thread": one["cell_updates_per_s"],
+            "sample": "CPU oracle (oracle/ramses_oracle.c, the reference's per-oct algorithm, OpenMP over octs) "
+                      "on the same turbulence problem at %d^3 cells: %d full steps (courant_fine + godunov_fine) "
+                      "in %.1f s on %d threads; %d steps in %.1f s on 1 thread (one_thread); host CPU: %s"
+                      % (n, allt["steps"], allt["seconds"], threads, one["steps"], one["seconds"], cpu_model())}
+
+
+def valu_ceiling(shape, ncells, kernel_ms):
+    """The FP64 kernel's VALU-issue ceiling from the committed counter summary
+    (when it was measured on this brick shape): every SIMD issuing one wave64
+    VALU instruction per 4 cycles at the 2.4 GHz peak engine clock (1024
+    SIMDs), divided by the counted VALU wave-instructions per cell-update."""
+    path = os.path.join(ROOT, "profiles", "godunov_fp64_turb256_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    if d.get("brick_shape") != "%dx%dx%d" % shape:
+        return None
+    ipc = d["valu_wave_instructions_per_cell"]
+    ceiling = SIMDS * CLOCK_HZ / 4.0 / ipc
+    achieved = ncells / (kernel_ms * 1e-3)
+    return {"valu_wave_instructions_per_cell": ipc, "ceiling_cell_updates_per_s": ceiling,
+            "achieved_cell_updates_per_s": achieved, "frac": achieved / ceiling,
+            "issue_busy_measured": d.get("valu_issue_busy"),
+            "source": "profiles/godunov_fp64_turb256_counters.json (SQ_INSTS_VALU per dispatch / cells)"}
 
 
 def load_traffic(shape):
@@ -123,21 +165,95 @@ def load_traffic(shape):
     return d.get("hbm_bytes_per_launch")
 
 
-def run_gpu(args, p, cells, mesh, precision, brick=None):
+def run_gpu(args, p, cells, mesh, precision, brick=None, kernel_steps=10):
+    """Warm-up steps, then the K timed steps replayed as the captured hipGraph
+    of a step pair (captured during the warm-up; --eager times them launch by
+    launch), then a separate eager pass of kernel_steps steps whose Godunov
+    launches are timed with HIP events on the library stream."""
     from ramses2_amd import UniformLevel, _lib
     from ramses2_amd.hydro import HipHydro
     if brick is not None:
         HipHydro.set_brick_shape(*brick)
     lev = UniformLevel(mesh, cells, p)
+    graph = not args.eager
     if args.warmup:
-        lev.run_device(args.warmup, precision=precision)
+        lev.run_device(max(args.warmup, 2) if graph else args.warmup, precision=precision, use_graph=graph)
     _lib.call("rh_device_sync")
     t0 = time.perf_counter()
-    dt_last, t_tot, sums, kms = lev.run_device(args.steps, precision=precision, use_graph=args.graph,
-                                                time_kernel=not args.graph)
+    dt_last, t_tot, sums, _ = lev.run_device(args.steps, precision=precision, use_graph=graph)
     _lib.call("rh_device_sync")
     el = time.perf_counter() - t0
+    _, _, _, kms = lev.run_device(kernel_steps, precision=precision, time_kernel=True)
     return lev, el, kms, dt_last, t_tot, sums
+
+
+def precision_study(args, p, cells, mesh, bricks, nsteps=100):
+    """The turbulence problem run nsteps steps in FP64 and in mixed precision
+    from the same initial state, each with its own Courant dt: L1 of each
+    conserved variable against FP64, and the drift of the total mass and
+    energy over the run in each precision."""
+    from ramses2_amd import UniformLevel
+    from ramses2_amd.hydro import HipHydro
+    out, states = {}, {}
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        HipHydro.set_brick_shape(*bricks[prec])
+        lev = UniformLevel(mesh, cells, p)
+        c0 = lev.courant()
+        _, t_tot, _, _ = lev.run_device(nsteps, precision=prec, use_graph=True)
+        states[name] = lev.state()
+        c1 = lev.courant()  # the same FP64 reduction as c0
+        out[name] = {"t_end": t_tot, "mass_drift": float((c1["mass"] - c0["mass"]) / c0["mass"]),
+                     "energy_drift": float((c1["ekin"] - c0["ekin"]) / c0["ekin"])}
+    names = ["rho", "mom_x", "mom_y", "mom_z", "E"]
+    u64, um = states["fp64"], states["mixed"]
+    out["mixed"]["l1_rel_vs_fp64"] = {names[v]: float(np.abs(um[:, v] - u64[:, v]).sum() / np.abs(u64[:, v]).sum())
+                                      for v in range(5)}
+    out["steps"] = nsteps
+    out["note"] = ("L1 = sum|mixed - fp64| / sum|fp64| over all cells after %d steps, each run with its own "
+                   "Courant dt; drift = relative change of the total over the run (periodic box)" % nsteps)
+    return out
+
+
+def sedov_uniform(args):
+    """Uniform 3D Sedov blast, 128^3 (levelmin = levelmax = 7) on the periodic
+    unit box, gamma 1.4, ambient rho 1, p 1e-5, E = 1 in the central 2^3
+    cells, MonCen, HLLC, Courant 0.5, sedov_steps steps in FP64 on the GPU;
+    the CPU oracle on the same problem at all host threads and at one."""
+    from ramses2_amd import HydroParams, UniformLevel, UniformMesh, _lib, ic
+    from ramses2_amd.autotune import DEFAULT_SHAPE
+    from ramses2_amd.hydro import HipHydro
+    n = 128
+    mesh = UniformMesh(3, 7)
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
+    cells = ic.sedov_3d(n)
+    ncells = mesh.noct * 8
+    HipHydro.set_brick_shape(*DEFAULT_SHAPE)
+    lev = UniformLevel(mesh, cells, p)
+    lev.run_device(2, use_graph=True)
+    _lib.call("rh_device_sync")
+    t0 = time.perf_counter()
+    _, t_tot, _, _ = lev.run_device(args.sedov_steps, use_graph=True)
+    _lib.call("rh_device_sync")
+    el = time.perf_counter() - t0
+    _, _, _, kms = lev.run_device(10, time_kernel=True)
+    out = {"workload": "3D Sedov blast, uniform 128^3 (levelmin=levelmax=7), periodic unit box, gamma 1.4, "
+                       "rho 1, p 1e-5, E=1 in the central 2^3 cells, MonCen, HLLC, Courant 0.5",
+           "steps": args.sedov_steps, "t_end": t_tot + 0.0,
+           "gpu": {"cell_updates_per_s": ncells * args.sedov_steps / el, "ms_per_step": el / args.sedov_steps * 1e3,
+                   "godunov_kernel_ms": kms,
+                   "hbm_gbs_algorithmic": BYTES_PER_CELL * ncells / (kms * 1e-3) / 1e9 if kms else None,
+                   "brick_shape": "%dx%dx%d" % DEFAULT_SHAPE}}
+    if not args.no_cpu:
+        out["cpu"] = {}
+        for threads in (len(os.sched_getaffinity(0)), 1):
+            r = cpu_rate(mesh, cells, p, threads, args.cpu_seconds / 3)
+            out["cpu"]["threads_%d" % threads] = r
+        out["gpu_over_cpu_all_threads"] = out["gpu"]["cell_updates_per_s"] / out["cpu"][
+            "threads_%d" % len(os.sched_getaffinity(0))]["cell_updates_per_s"]
+    print("bench: Sedov 128^3 FP64 %.3f ms/step (%.3g cell-updates/s, kernel %.3f ms, %.0f GB/s algorithmic)"
+          % (out["gpu"]["ms_per_step"], out["gpu"]["cell_updates_per_s"], kms or -1,
+             out["gpu"]["hbm_gbs_algorithmic"] or -1), file=sys.stderr, flush=True)
+    return out
 
 
 def run_amr(args):
@@ -179,6 +295,8 @@ def run_amr(args):
         h.run_device(2, precision=prec, use_graph=True)
         dt, tt, sums, ms = h.run_device(args.amr_steps, precision=prec, use_graph=True)
         states[name] = (h.state(), sums)
+        if prec == 0:
+            out["flags_after_run"] = amr_flags(h, am, p)
         out[name] = {"ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3), "steps": args.amr_steps}
         print("bench: AMR %s %.3f ms/step (%.3g cell-updates/s)" % (name, ms, cells / (ms * 1e-3)),
               file=sys.stderr, flush=True)
@@ -211,6 +329,24 @@ def run_amr(args):
           % (ms_d, out["fp64"]["speedup_vs_dropin"], out["mixed"]["speedup_vs_fp64"]), file=sys.stderr, flush=True)
     HipHydro.set_brick_shape(*DEFAULT_SHAPE)
     return out
+
+
+def amr_flags(h, am, p, err=0.1):
+    """The refinement criterion (rh_hydro_flag: relative density and pressure
+    gradients > err) on the device state after the timed steps, per level:
+    cells flagged, flagged cells without children (a refine step would split
+    them) and refined cells not flagged (candidates for derefinement)."""
+    t0 = time.perf_counter()
+    out = {}
+    for l in am.levels:
+        f, n = h.hh.hydro_flag(am.head[l], am.noct[l], p, err, err)
+        ref = am.refined[am.head[l] - 1:am.head[l] - 1 + am.noct[l]] != 0
+        out[str(l)] = {"flagged": int(n), "flagged_leaf": int(((f != 0) & ~ref).sum()),
+                       "refined_unflagged": int((ref & (f == 0)).sum())}
+    ms = (time.perf_counter() - t0) * 1e3
+    print("bench: AMR flags from the evolved state (err_grad_d = err_grad_p = %g): %s (%.1f ms)"
+          % (err, out, ms), file=sys.stderr, flush=True)
+    return {"err_grad_d": err, "err_grad_p": err, "per_level": out, "ms_all_levels": ms}
 
 
 def run_serving(args):
@@ -350,7 +486,6 @@ def main():
                 file=sys.stderr, flush=True)
 
     lev64, el64, kms64, dt64, t64, sums64 = run_gpu(args, p, cells, mesh, precision=0, brick=bricks[0])
-    u64 = lev64.state()
     value = ncells * args.steps / el64
     ms_per_step = el64 / args.steps * 1e3
     print("bench: fp64 %.3f ms/step, godunov kernel %.3f ms" % (ms_per_step, kms64 or -1), file=sys.stderr,
@@ -359,17 +494,17 @@ def main():
     mixed = None
     if not args.no_mixed:
         levm, elm, kmsm, dtm, tm, sumsm = run_gpu(args, p, cells, mesh, precision=1, brick=bricks[1])
-        um = levm.state()
-        l1 = {}
-        names = ["rho", "mom_x", "mom_y", "mom_z", "E"]
-        for v in range(5):
-            l1[names[v]] = float(np.abs(um[:, v] - u64[:, v]).sum() / np.abs(u64[:, v]).sum())
         mixed = {"value": ncells * args.steps / elm, "ms_per_step": elm / args.steps * 1e3,
                  "speedup_vs_fp64": el64 / elm, "godunov_kernel_ms": kmsm,
-                 "l1_rel_vs_fp64": l1,
-                 "l1_note": "after %d steps, each run with its own Courant dt" % (args.warmup + args.steps),
-                 "mass_drift": float((sumsm[0] - sums64[0]) / sums64[0])}
-        print("bench: mixed %.3f ms/step (x%.2f)" % (mixed["ms_per_step"], mixed["speedup_vs_fp64"]),
+                 "kernel_speedup_vs_fp64": (kms64 / kmsm) if (kms64 and kmsm) else None}
+        print("bench: mixed %.3f ms/step (x%.2f), godunov kernel %.3f ms"
+              % (mixed["ms_per_step"], mixed["speedup_vs_fp64"], kmsm or -1), file=sys.stderr, flush=True)
+        study = precision_study(args, p, cells, mesh, bricks, nsteps=args.study_steps)
+        mixed["precision_study"] = study
+        print("bench: precision study %d steps: L1 mixed vs FP64 %s; drift mass %.2g/%.2g energy %.2g/%.2g "
+              "(FP64/mixed)" % (study["steps"], {k: "%.2g" % v for k, v in study["mixed"]["l1_rel_vs_fp64"].items()},
+                                study["fp64"]["mass_drift"], study["mixed"]["mass_drift"],
+                                study["fp64"]["energy_drift"], study["mixed"]["energy_drift"]),
               file=sys.stderr, flush=True)
 
     roof = None
@@ -380,6 +515,14 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": "k_godunov3d<%d,%d,%d,double> (FP64)" % bricks[0], "kernel_ms": kms64,
                 "bytes_per_launch": BYTES_PER_CELL * ncells}
+        valu = valu_ceiling(bricks[0], ncells, kms64)
+        if valu:
+            roof["valu"] = valu
+            roof["limiter"] = "valu-issue" if valu["frac"] > roof["frac"] else "hbm"
+
+    sedov = None
+    if not args.no_sedov:
+        sedov = sedov_uniform(args)
 
     amr = None
     if not args.no_amr:
@@ -416,6 +559,7 @@ def main():
         "mixed": mixed,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "sedov_128": sedov,
         "amr": amr,
         "serving": serving,
     }

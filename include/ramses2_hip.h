@@ -142,6 +142,29 @@ int rh_upload(int head_idx, int num_octs, int internal_energy, double gamma, dou
  * them from its module variables (gpu/gpu_refine.cuf:2231 make_cache_octs). */
 int rh_set_interpolation(int interpol_var, int interpol_type);
 
+/* ---- Refinement flags (the flagging step of the refine path) -------------
+ * flag1(1:8, 1:ngridmax) int32 on the device, as m%flag1 (amr/amr_commons.f90).
+ * Replaces mtl_upload_flag1 (metal/metal_interface.f90:59-63): host -> device. */
+int rh_upload_flag1(const int32_t *flag1, int ngridmax);
+/* Device -> host copy of flag1 (the reverse of rh_upload_flag1; the Metal
+ * bridge's flag1 is host-shared memory, metal/metal_bridge.mm:1016). */
+int rh_transfer_flag1_host(int32_t *flag1, int ngridmax);
+/* Replaces mtl_reset_flag1 (metal/metal_interface.f90:200-203): flag1 = 0 for
+ * the cells of octs head_idx..head_idx+num_octs-1. */
+int rh_reset_flag1(int head_idx, int num_octs);
+/* Replaces mtl_count_flag1 (metal/metal_interface.f90:218-222): the number of
+ * flagged cells of the octs (an int function there, an output here). */
+int rh_count_flag1(int head_idx, int num_octs, int32_t *count);
+/* Replaces mtl_hydro_flag (metal/metal_interface.f90:225-241), FP64: the
+ * gradient criteria of hydro_flag / hydro_refine (hydro/hydro_flag.f90:7-159,
+ * hydro/godunov_utils.f90:6-227) on the level's current uold: flag1 = 1 where
+ * 2 max(|d_R - d|/(d_R + d + floor_d), |d - d_L|/(d + d_L + floor_d)) >
+ * err_grad_d, or the same of the pressure > err_grad_p (-1 disables a
+ * criterion), d_L/d_R the face neighbours in each direction (from the coarser
+ * level where the level has no oct there).  Set flags stay set.  3D only. */
+int rh_hydro_flag(int head_idx, int num_octs, double gamma, double smallr, double smallc2, double err_grad_d,
+                  double err_grad_p, double floor_d, double floor_p);
+
 /* Replaces mtl_device_sync (metal/metal_interface.f90:529-531). */
 int rh_device_sync(void);
 

@@ -26,10 +26,29 @@ int rh_set_precision(int precision);
  * as it stores it, the FP32 kernel in an epilogue that re-reads its cells). */
 int rh_set_fused_courant(int mode);
 
+/* Deterministic coarse-fine refluxes: 1 = each face between a level cell and
+ * an interpolated ghost cell stores its flux in a slot of its own (24 per
+ * ghost oct), and a fold adds the slots to the parent cell in a fixed order
+ * after the level's launch, so that repeated runs give the same bits (the
+ * reference adds them serially, hydro/godunov_fine.f90:894-1052); 0 (the
+ * default) = FP64 atomics, whose order varies from run to run.  Applies to
+ * rh_godunov and to the next rh_amr_prepare. */
+int rh_set_deterministic(int on);
+
 /* Oct-brick shape (octs per brick in x, y, z) used by the next rh_build_nbor
  * for 3D levels; the autotuner (ramses2_amd/autotune.py) picks it. */
 int rh_set_brick_shape(int bx, int by, int bz);
 int rh_get_brick_shape(int32_t *bx, int32_t *by, int32_t *bz);
+
+/* Copy one of a level's device index tables to the host (for checking the
+ * indexing against a brute-force key lookup).  which: 0 neighbour table
+ * [noct][3^ndim] (oct index, -(reference oct) for a boundary ghost, 0 none);
+ * 1 father [noct][2] (parent oct, cell 1..2^ndim); 2 brick origins
+ * [nbricks][3]; 3 brick windows [nbricks][(bx+2)(by+2)(bz+2)]; 4 ghost octs
+ * of the level [nghost][4] (oct index, key x, y, z); 5 their parent cells
+ * [nghost][2].  *len receives the number of int32 values of the table; the
+ * copy happens when capacity >= *len (else RH_ERR_ARG). */
+int rh_get_level_table(int ilevel, int which, int32_t *out, int64_t capacity, int64_t *len);
 
 /* HIP-event timer on the library stream: milliseconds between start and stop. */
 int rh_timer_start(void);
@@ -39,10 +58,18 @@ int rh_timer_stop(float *ms);
  * courant_fine (dt reduced on the device) -> godunov_fine with set_unew and
  * set_uold folded in (unew = uold + dF, then the two arrays swap roles).
  * dt_fixed > 0 imposes dt instead of the Courant value.  dt_last, t_total and
- * the conservation sums of the last Courant pass are returned when non-NULL.
- * use_graph != 0 replays the step pair as a captured hipGraph.  kernel_ms, when
- * non-NULL (and use_graph == 0), receives the mean duration of the Godunov
- * launches, timed with HIP events on the library stream inside the steps. */
+ * the conservation sums (mass, total energy, internal energy, 0) of the last
+ * Courant pass are returned when non-NULL.  With the Courant pass in the
+ * Godunov kernel (rh_set_fused_courant, the default) that pass is over the
+ * state after the last step, and dt_last is the dt the next step would take;
+ * with a separate Courant kernel it is over the state before the last step,
+ * and dt_last is the dt that step took.
+ * use_graph != 0 replays the step pair as a captured hipGraph; the graph is
+ * kept and replayed again by the next call with the same arguments on the
+ * same arrays and tables (rh_build_nbor, rh_alloc_amr and rh_finalize drop
+ * it).  kernel_ms, when non-NULL (and use_graph == 0), receives the mean
+ * duration of the Godunov launches, timed with HIP events on the library
+ * stream inside the steps. */
 int rh_level_run(int head_idx, int num_octs, int ilevel, double dx, double gamma, double smallr,
                  double smallc2, double courant_factor, const double g[3], int slope, int riemann,
                  double dt_fixed, int nsteps, int use_graph, double *dt_last, double *t_total,

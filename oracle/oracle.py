@@ -244,6 +244,20 @@ class OracleMesh:
         c = lib().ora_probe_get_grid(ctypes.byref(self.m), lev, k, _ptr(u, _dp), _ptr(r, _u8p))
         return c, u, r
 
+    def hydro_flag(self, ilevel, gamma, smallr, err_grad_d, err_grad_p, floor_d=1e-10, floor_p=1e-10, flag1=None):
+        """ora_hydro_flag: flag1 (ngrid, 2^ndim) int32, OR-ed into the given
+        array (zeros when None); returns (flag1, newly flagged count)."""
+        ttd = 2 ** self.ndim
+        f = np.zeros((self.m.ngrid, ttd), dtype=np.int32) if flag1 is None else np.ascontiguousarray(flag1, np.int32)
+        L = lib()
+        L.ora_hydro_flag.restype = ctypes.c_int
+        n = L.ora_hydro_flag(ctypes.byref(self.m), ilevel, ctypes.c_double(gamma), ctypes.c_double(smallr),
+                             ctypes.c_double(err_grad_d), ctypes.c_double(err_grad_p), ctypes.c_double(floor_d),
+                             ctypes.c_double(floor_p), _ptr(f, _ip))
+        if n < 0:
+            raise RuntimeError("ora_hydro_flag: a neighbour's parent cell is missing")
+        return f, n
+
     def godunov_uniform_omp(self, ilevel, dt, p, octmap, nthreads):
         octmap = np.ascontiguousarray(octmap, dtype=np.int32)
         rc = lib().ora_godunov_uniform_omp(ctypes.byref(self.m), ilevel, dt, ctypes.byref(p),
@@ -351,6 +365,17 @@ class RefLevel:
         self.probed = True
         c = self.L.reflev_get_grid(lev, k, _ptr(u, _dp), r.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)))
         return c, u, r.astype(np.uint8)
+
+    def hydro_flag(self, ilevel, err_grad_d, err_grad_p, floor_d=1e-10, floor_p=1e-10):
+        """The reference's hydro_flag on level ilevel (flag1 cleared first):
+        flag1 (ngrid, 2^ndim) int32."""
+        self._cache_ok()
+        ttd = 2 ** self.ndim
+        f = np.zeros((self.ngrid, ttd), dtype=np.int32)
+        d = ctypes.c_double
+        self.L.reflev_hydro_flag(ilevel, d(err_grad_d), d(err_grad_p), d(floor_d), d(floor_p), self.ngrid,
+                                 _ptr(f, _ip))
+        return f
 
     def set_unew(self, ilevel):
         self.L.reflev_set_unew(ilevel)

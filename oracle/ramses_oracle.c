@@ -852,8 +852,14 @@ static void wrap_key(const ora_mesh_t *m, int lev, int64_t *key)
         }
 }
 
-/* Fortran integer division (truncation toward zero) */
-static int64_t fdiv2(int64_t a) { return a / 2; }
+/* Parent key of a cell key: key/2 as the reference divides (Fortran integer
+ * division, amr/nbors_utils.f90:180,200,241) for the non-negative keys of its
+ * meshes, whose non-periodic boxes start at box_ckey_min >= 1 with boundary
+ * octs below (amr/init_amr.f90:255-283).  Floor division extends this to boxes
+ * that start at key 0, where a neighbour key -1 belongs to the boundary oct
+ * -1 (cell bit 1), exactly as in the reference's mesh shifted by an even
+ * number of keys; truncation would give parent 0 and cell bit -1. */
+static int64_t fdiv2(int64_t a) { return a >= 0 ? a / 2 : -((1 - a) / 2); }
 
 /* ------------------------------------------------------------------------ */
 /* godfine1, hydro/godunov_fine.f90:244-1342 (nsubgrid = 1)                  */
@@ -904,13 +910,19 @@ static int godfine1(const lookup_t *L, int ind_grid, int ilevel, double dt, ora_
                         }
                         icell = 1;
                         for (int d = 0; d < ndim; d++) icell += (1 << d) * ii[d];
+                        /* a stencil oct outside the box whose reference oct is
+                         * missing has its parent among the coarse level's
+                         * boundary octs (get_grid builds one in the cache,
+                         * amr/nbors_utils.f90:580-640): igrid = -1, values gp */
                         ghost_t gp;
                         igrid = get_grid(L, ilevel - 1, kf, &gp);
-                        if (igrid <= 0) return 1; /* GODUNOV: parent_cell should exist */
+                        if (igrid == 0) return 1; /* GODUNOV: parent_cell should exist */
                         /* get_twondim_nbor_parent_cell (amr/nbors_utils.f90:149-217) */
                         double u1[7 * ORA_NVAR];
                         int n1 = 2 * ndim + 1;
-                        for (int ivar = 1; ivar <= nvar; ivar++) u1[0 + n1 * (ivar - 1)] = UOCT(m->uold, nvar, igrid, ivar, icell, ttd);
+                        for (int ivar = 1; ivar <= nvar; ivar++)
+                            u1[0 + n1 * (ivar - 1)] = igrid > 0 ? UOCT(m->uold, nvar, igrid, ivar, icell, ttd)
+                                                                : gp.u[(ivar - 1) * ttd + (icell - 1)];
                         for (int inbor = 1; inbor <= 2 * ndim; inbor++) {
                             int d = (inbor - 1) / 2;
                             int64_t kn[3] = {key[0], key[1], key[2]};
@@ -935,7 +947,9 @@ static int godfine1(const lookup_t *L, int ind_grid, int ilevel, double dt, ora_
                         use_interp = 1;
                     }
                 }
-                childloc[OI(i1, j1, k1)] = ichild;
+                /* a boundary ghost oct is an oct of the cache in the reference
+                 * (childloc > 0): no reflux through its faces */
+                childloc[OI(i1, j1, k1)] = use_ghost ? -1 : ichild;
                 gridloc[OI(i1, j1, k1)] = igrid;
                 cellloc[OI(i1, j1, k1)] = icell;
                 /* 2^ndim cells of this oct into the stencil */
@@ -1020,6 +1034,9 @@ static int godfine1(const lookup_t *L, int ind_grid, int ilevel, double dt, ora_
                 int ichild = childloc[OI(i1, j1, k1)];
                 if (ichild != 0) continue;
                 int igrid = gridloc[OI(i1, j1, k1)], icell = cellloc[OI(i1, j1, k1)];
+                /* parent cell in a boundary oct of the coarse level: a cache oct
+                 * that is never flushed back, so the reflux is dropped */
+                if (igrid < 0) continue;
                 int k2lo = 0, k2hi = ndim > 2 ? 1 : 0, j2lo = 0, j2hi = ndim > 1 ? 1 : 0, i2lo = 0, i2hi = 1;
                 if (side == 0) { i2hi -= i0; j2hi -= j0; k2hi -= k0; }
                 else { i2lo += i0; j2lo += j0; k2lo += k0; }
@@ -1094,6 +1111,117 @@ void ora_set_uold(const ora_mesh_t *m, int ilevel)
     level_range(m, ilevel, &head, &noct);
     size_t per = (size_t)m->nvar << m->ndim;
     memcpy(m->uold + (size_t)(head - 1) * per, m->unew + (size_t)(head - 1) * per, sizeof(double) * per * noct);
+}
+
+/* hydro_refine, hydro/godunov_utils.f90:6-227, density and pressure criteria
+ * (no MHD, no non-thermal energies, no passive scalars), in its expression
+ * order; ug/um/ud are modified in place as there. */
+static int hydro_refine_dp(double gamma, double smallr, double err_grad_d, double err_grad_p, double floor_d,
+                           double floor_p, double *ug, double *um, double *ud, int ok)
+{
+    ug[0] = ug[0] > smallr ? ug[0] : smallr;
+    um[0] = um[0] > smallr ? um[0] : smallr;
+    ud[0] = ud[0] > smallr ? ud[0] : smallr;
+    for (int d = 1; d <= 3; d++) {
+        ug[d] = ug[d] / ug[0];
+        um[d] = um[d] / um[0];
+        ud[d] = ud[d] / ud[0];
+    }
+    double eking = 0.0, ekinm = 0.0, ekind = 0.0;
+    for (int d = 1; d <= 3; d++) {
+        eking = eking + 0.5 * ug[0] * (ug[d] * ug[d]);
+        ekinm = ekinm + 0.5 * um[0] * (um[d] * um[d]);
+        ekind = ekind + 0.5 * ud[0] * (ud[d] * ud[d]);
+    }
+    /* (gamma-1)*(E - ekin - emag - erad) with emag = erad = 0 */
+    double pg = (gamma - 1.0) * (ug[4] - eking - 0.0 - 0.0);
+    double pm = (gamma - 1.0) * (um[4] - ekinm - 0.0 - 0.0);
+    double pd = (gamma - 1.0) * (ud[4] - ekind - 0.0 - 0.0);
+    if (err_grad_d >= 0.0) {
+        double dg = ug[0], dm = um[0], dd = ud[0];
+        double a = fabs((dd - dm) / (dd + dm + floor_d)), b = fabs((dm - dg) / (dm + dg + floor_d));
+        double error = 2.0 * (a > b ? a : b);
+        ok = ok || error > err_grad_d;
+    }
+    if (err_grad_p >= 0.0) {
+        double a = fabs((pd - pm) / (pd + pm + floor_p)), b = fabs((pm - pg) / (pm + pg + floor_p));
+        double error = 2.0 * (a > b ? a : b);
+        ok = ok || error > err_grad_p;
+    }
+    return ok;
+}
+
+/* hydro_flag, hydro/hydro_flag.f90:7-159 (gradient criteria) */
+int ora_hydro_flag(const ora_mesh_t *m, int ilevel, double gamma, double smallr, double err_grad_d,
+                   double err_grad_p, double floor_d, double floor_p, int32_t *flag1)
+{
+    if (err_grad_d == -1.0 && err_grad_p == -1.0) return 0;
+    omap_t h;
+    omap_build(&h, m);
+    const int ndim = m->ndim, ttd = 1 << ndim, nvar = m->nvar;
+    lookup_t L = {m, &h, ttd};
+    int head, noct;
+    level_range(m, ilevel, &head, &noct);
+    int nflag = 0, rc = 0;
+    for (int igrid = head; igrid < head + noct && !rc; igrid++) {
+        for (int ind = 1; ind <= ttd; ind++) {
+            /* cell key at level ilevel+1 (iii of hydro_flag: bits of ind-1) */
+            int64_t hk[3] = {0, 0, 0};
+            for (int d = 0; d < ndim; d++)
+                hk[d] = 2 * (int64_t)m->ckey[(size_t)(igrid - 1) * 3 + d] + (((ind - 1) >> d) & 1);
+            int ok = 0;
+            /* the 2 ndim neighbour cells (shift: -x +x -y +y -z +z): the cell
+             * of level ilevel that holds it, else the one of level ilevel-1 */
+            int gn[6];
+            double un[6][5];
+            for (int inb = 0; inb < 2 * ndim && !rc; inb++) {
+                int64_t kn[3] = {hk[0], hk[1], hk[2]};
+                kn[inb / 2] += (inb % 2 == 0) ? -1 : 1;
+                wrap_key(m, ilevel + 1, kn);
+                int64_t kp[3] = {0, 0, 0};
+                int icell = 1;
+                for (int d = 0; d < ndim; d++) {
+                    kp[d] = fdiv2(kn[d]);
+                    icell += (1 << d) * (int)(kn[d] - 2 * kp[d]);
+                }
+                ghost_t g;
+                int c = get_grid(&L, ilevel, kp, &g);
+                if (c == 0) {
+                    int64_t kq[3] = {0, 0, 0};
+                    icell = 1;
+                    for (int d = 0; d < ndim; d++) {
+                        int64_t k2 = fdiv2(kn[d]);
+                        kq[d] = fdiv2(k2);
+                        icell += (1 << d) * (int)(k2 - 2 * kq[d]);
+                    }
+                    c = get_grid(&L, ilevel - 1, kq, &g);
+                    if (c == 0) {
+                        rc = 1;
+                        break;
+                    }
+                }
+                gn[inb] = c;
+                for (int v = 1; v <= nvar; v++)
+                    un[inb][v - 1] = c > 0 ? UOCT(m->uold, nvar, c, v, icell, ttd) : g.u[(v - 1) * ttd + (icell - 1)];
+            }
+            if (rc) break;
+            (void)gn;
+            for (int d = 0; d < ndim; d++) {
+                double ug[5], um[5], ud[5];
+                for (int v = 1; v <= nvar; v++) {
+                    ug[v - 1] = un[2 * d][v - 1];
+                    um[v - 1] = UOCT(m->uold, nvar, igrid, v, ind, ttd);
+                    ud[v - 1] = un[2 * d + 1][v - 1];
+                }
+                ok = hydro_refine_dp(gamma, smallr, err_grad_d, err_grad_p, floor_d, floor_p, ug, um, ud, ok);
+            }
+            int32_t *f = &flag1[(size_t)(igrid - 1) * ttd + (ind - 1)];
+            if (*f == 0 && ok) nflag++;
+            if (ok) *f = 1;
+        }
+    }
+    omap_free(&h);
+    return rc ? -1 : nflag;
 }
 
 /* upload_fine, hydro/upload.f90:74-283 */

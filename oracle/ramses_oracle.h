@@ -107,6 +107,15 @@ int ora_upload_fine(const ora_mesh_t *m, int ilevel, double smallr);
  * u[nvar][twotondim] and refined[twotondim] receive that oct's contents.    */
 int ora_probe_get_grid(const ora_mesh_t *m, int lev, const int64_t *key, double *u, uint8_t *refined);
 
+/* hydro_flag, hydro/hydro_flag.f90:7-159, with the density and pressure
+ * criteria of hydro_refine (hydro/godunov_utils.f90:6-227; err_grad_u and the
+ * other criteria off, no Lagrangian mass criterion): sets flag1[oct][2^ndim] to 1
+ * for each cell of level ilevel whose criterion holds (flag1 is OR-ed, not
+ * cleared).  Returns the number of cells newly flagged, or -1 when a
+ * neighbour's parent cell is missing. */
+int ora_hydro_flag(const ora_mesh_t *m, int ilevel, double gamma, double smallr, double err_grad_d,
+                   double err_grad_p, double floor_d, double floor_p, int32_t *flag1);
+
 /* Uniform-level fast path for the CPU baseline: same arithmetic as
  * ora_godunov_fine on a periodic uniform level whose octs are addressed
  * through a dense (ix,iy,iz) -> oct map, run with OpenMP over octs.          */

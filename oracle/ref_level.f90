@@ -26,6 +26,7 @@ module ref_level
   use godunov_fine_module, only: godunov_fine, set_unew, set_uold
   use courant_fine_module, only: courant_fine
   use upload_module, only: upload_fine
+  use hydro_flag_module, only: hydro_flag
   use cache, only: open_cache
   use nbors_utils, only: get_grid
   use boundaries, only: init_bound_refine
@@ -270,6 +271,27 @@ contains
        end do
     end if
   end function reflev_get_grid
+
+  ! hydro_flag (hydro/hydro_flag.f90:7-159, the gradient criteria of
+  ! hydro_refine, hydro/godunov_utils.f90:6-227) on the level's cells, flag1
+  ! cleared first; flag1 of the first ngrid octs is copied out.  Periodic
+  ! meshes only (the routine closes its cache, see reflev_godunov).
+  subroutine reflev_hydro_flag(ilevel, err_grad_d, err_grad_p, floor_d, floor_p, ngrid, flag1) &
+       & bind(C, name="reflev_hydro_flag")
+    integer(c_int), value :: ilevel, ngrid
+    real(c_double), value :: err_grad_d, err_grad_p, floor_d, floor_p
+    integer(c_int32_t), dimension(twotondim, ngrid) :: flag1
+    s%r%err_grad_d = err_grad_d
+    s%r%err_grad_p = err_grad_p
+    s%r%floor_d = floor_d
+    s%r%floor_p = floor_p
+    s%r%m_refine = -1.0
+    s%r%poisson = .false.
+    mesh%flag1 = 0
+    s%g%nflag = 0
+    call hydro_flag(s, ilevel)
+    flag1 = int(mesh%flag1(:, 1:ngrid), c_int32_t)
+  end subroutine reflev_hydro_flag
 
   ! Copy the first ngrid octs of uold and unew out.
   subroutine reflev_get(ngrid, uold, unew) bind(C, name="reflev_get")

@@ -11,6 +11,7 @@ SIGNATURES = {
     "rh_set_precision": [_c],
     "rh_set_fused_courant": [_c],
     "rh_set_brick_shape": [_c, _c, _c],
+    "rh_set_deterministic": [_c],
     "rh_get_brick_shape": [_ip, _ip, _ip],
     "rh_timer_start": [],
     "rh_timer_stop": [_fp],
@@ -31,6 +32,7 @@ SIGNATURES = {
     "rh_job_prepare": [_c, _c, _c, _d, _d, _d, _d, _d, _dp, _c, _c],
     "rh_batch_step": [_ip, _c, _c, _c],
     "rh_job_status": [_c, _dp],
+    "rh_get_level_table": [_c, _c, _ip, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)],
 }
 
 

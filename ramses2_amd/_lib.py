@@ -36,6 +36,11 @@ SIGNATURES = {
     "rh_device_sync": [],
     "rh_upload": [_c, _c, _c, _d, _d, _d],
     "rh_set_interpolation": [_c, _c],
+    "rh_upload_flag1": [_ip, _c],
+    "rh_transfer_flag1_host": [_ip, _c],
+    "rh_reset_flag1": [_c, _c],
+    "rh_count_flag1": [_c, _c, _ip],
+    "rh_hydro_flag": [_c, _c, _d, _d, _d, _d, _d, _d, _d],
     "rh_last_error": None,
 }
 

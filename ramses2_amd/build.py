@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libramses2_hip.so"
-SOURCES = ["rh_abi.hip", "rh_godunov.hip", "rh_ext.hip", "rh_serve.hip", "rh_amr.hip"]
+SOURCES = ["rh_abi.hip", "rh_godunov.hip", "rh_ext.hip", "rh_serve.hip", "rh_amr.hip", "rh_flag.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RH_OFFLOAD_ARCH", "gfx950")
 

@@ -11,6 +11,7 @@
 
 #include "../../include/ramses2_hip.h"
 #include "rh_godunov.h"
+#include "rh_mesh.h"
 #include "rh_state.h"
 
 namespace rh {
@@ -107,43 +108,6 @@ static int fail(int code, const std::string &msg)
     return code;
 }
 
-// ---------------------------------------------------------------------------
-// Device hash: open addressing, linear probing, 64-bit keys.
-// key = key_off(lev) + ix + iy*nx + iz*nx^2 (gpu/gpu_hash.cuf:240-245), nx = 2^(lev-1).
-// ---------------------------------------------------------------------------
-static constexpr uint64_t kEmpty = ~0ull;
-
-__device__ __forceinline__ uint64_t hmix(uint64_t k)
-{
-    k ^= k >> 31;
-    k *= 0x7fb5d329728ea185ull;
-    k ^= k >> 27;
-    k *= 0x81dadef4bc2dd44dull;
-    k ^= k >> 33;
-    return k;
-}
-
-__device__ __forceinline__ uint64_t oct_key(int64_t koff, int nx, int ndim, int ix, int iy, int iz)
-{
-    uint64_t k = (uint64_t)koff + (uint64_t)ix;
-    if (ndim > 1) k += (uint64_t)iy * (uint64_t)nx;
-    if (ndim > 2) k += (uint64_t)iz * (uint64_t)nx * (uint64_t)nx;
-    return k;
-}
-
-__device__ __forceinline__ int hash_get(const uint64_t *__restrict__ keys, const int32_t *__restrict__ vals,
-                                        uint64_t cap, uint64_t key)
-{
-    uint64_t h = hmix(key) & (cap - 1);
-    for (uint64_t n = 0; n < cap; n++) {
-        const uint64_t k = keys[h];
-        if (k == key) return vals[h];
-        if (k == kEmpty) return 0;
-        h = (h + 1) & (cap - 1);
-    }
-    return 0;
-}
-
 struct LevelTab {
     int64_t key_off[64];
     int32_t nx[64];
@@ -173,43 +137,6 @@ __global__ void k_hash_insert(const int32_t *__restrict__ ckey, const int32_t *_
         h = (h + 1) & (cap - 1);
     }
     atomicOr(overflow, 1);
-}
-
-// Resolve the oct at (possibly out-of-domain) key k of a level.  Returns the
-// oct index, -(reference oct) for a boundary ghost, 0 if absent.  Periodic
-// directions wrap modulo the box (hydro/godunov_fine.f90:463-468).
-struct ResolveCtx {
-    const uint64_t *keys;
-    const int32_t *vals;
-    uint64_t cap;
-    int64_t koff;
-    int nx, ndim;
-    int boxmin[3], boxmax[3], periodic[3];
-};
-
-__device__ __forceinline__ int resolve_oct(const ResolveCtx &c, int k0, int k1, int k2)
-{
-    int k[3] = {k0, k1, k2};
-    for (int d = 0; d < c.ndim; d++) {
-        if (c.periodic[d]) {
-            const int n = c.boxmax[d] - c.boxmin[d];
-            int r = (k[d] - c.boxmin[d]) % n;
-            if (r < 0) r += n;
-            k[d] = c.boxmin[d] + r;
-        }
-    }
-    int out_dim = -1, side = 0;
-    for (int d = c.ndim - 1; d >= 0; d--) {
-        if (k[d] < c.boxmin[d]) { out_dim = d; side = 0; }
-        else if (k[d] >= c.boxmax[d]) { out_dim = d; side = 1; }
-    }
-    if (out_dim < 0) return hash_get(c.keys, c.vals, c.cap, oct_key(c.koff, c.nx, c.ndim, k[0], k[1], k[2]));
-    // boundary ghost: the oct one step back into the domain along out_dim
-    k[out_dim] += side == 0 ? 1 : -1;
-    for (int d = 0; d < c.ndim; d++)
-        if (k[d] < c.boxmin[d] || k[d] >= c.boxmax[d]) return 0;
-    const int r = hash_get(c.keys, c.vals, c.cap, oct_key(c.koff, c.nx, c.ndim, k[0], k[1], k[2]));
-    return r > 0 ? -r : 0;
 }
 
 // 3^ndim neighbour slots of each oct of the level (slot 1+i+3j+9k, 0-based here).
@@ -259,6 +186,7 @@ __global__ void k_make_ghosts(const int32_t *__restrict__ ckey, const int32_t *_
     const int o = head - 1 + (int)(i / nslot);
     const int s = (int)(i % nslot);
     int k[3] = {ckey[3 * o] + s % 3 - 1, ckey[3 * o + 1] + (s / 3) % 3 - 1, ckey[3 * o + 2] + s / 9 - 1};
+    int nout = 0;
     for (int d = 0; d < c.ndim; d++) {
         if (c.periodic[d]) {
             const int n = c.boxmax[d] - c.boxmin[d];
@@ -266,9 +194,14 @@ __global__ void k_make_ghosts(const int32_t *__restrict__ ckey, const int32_t *_
             if (r < 0) r += n;
             k[d] = c.boxmin[d] + r;
         }
-        if (k[d] < c.boxmin[d] || k[d] >= c.boxmax[d]) return;  // boundary ghosts are synthesised, not stored
+        nout += k[d] < c.boxmin[d] || k[d] >= c.boxmax[d];
     }
-    const uint64_t key = oct_key(c.koff, c.nx, c.ndim, k[0], k[1], k[2]);
+    // outside the box in one dimension (the boundary reference oct is absent,
+    // or the slot would not be empty): interpolated from the parent level's
+    // boundary octs, as the reference's get_grid leaves it to get_parent_cell
+    // (amr/nbors_utils.f90:580-640, hydro/godunov_fine.f90:480-560)
+    if (nout > 1 || (nout == 1 && c.ndim < 3)) return;
+    const uint64_t key = nout ? ext_key(c.nx, k[0], k[1], k[2]) : oct_key(c.koff, c.nx, c.ndim, k[0], k[1], k[2]);
     uint64_t h = hmix(key) & (c.cap - 1);
     for (uint64_t p = 0; p < c.cap; p++) {
         unsigned long long prev = atomicCAS((unsigned long long *)&keys[h], (unsigned long long)kEmpty,
@@ -322,10 +255,13 @@ __global__ void k_ghost_sources(const int32_t *__restrict__ gkey, int first_gi, 
                 kn[d] = fine.boxmin[d] + r;
             }
         }
+        // parent key and cell bit by floor division: the reference's integer
+        // division for its non-negative keys, and the boundary oct -1 (cell
+        // bit 1) for the key -1 next to a non-periodic box that starts at 0
         int kf[3] = {0, 0, 0}, ind = 1;
         for (int d = 0; d < nd; d++) {
-            kf[d] = kn[d] / 2;  // Fortran integer division
-            ind += (1 << d) * (kn[d] - 2 * kf[d]);
+            kf[d] = kn[d] >> 1;
+            ind += (1 << d) * (kn[d] & 1);
         }
         const int code = resolve_oct(coarse, kf[0], kf[1], kf[2]);
         gsrc[(gi * 7 + slot) * 2] = code;
@@ -334,9 +270,11 @@ __global__ void k_ghost_sources(const int32_t *__restrict__ gkey, int first_gi, 
         gsrckey[(gi * 7 + slot) * 3 + 1] = kf[1];
         gsrckey[(gi * 7 + slot) * 3 + 2] = kf[2];
         if (slot == 0) {
+            // a negative code is a boundary oct of the coarse level (a ghost
+            // beyond a non-periodic face): its refluxes are dropped
             gparent[2 * gi] = code;
             gparent[2 * gi + 1] = ind;
-            if (code <= 0) atomicOr(&counter[2], 1);
+            if (code == 0) atomicOr(&counter[2], 1);
         } else if (code == 0) {
             atomicOr(&counter[2], 2);
         }
@@ -353,8 +291,8 @@ __global__ void k_build_father(const int32_t *__restrict__ ckey, int head, int n
     const int o = head - 1 + i;
     int kf[3] = {0, 0, 0}, ind = 1;
     for (int d = 0; d < ndim; d++) {
-        kf[d] = ckey[3 * o + d] / 2;
-        ind += (1 << d) * (ckey[3 * o + d] - 2 * kf[d]);
+        kf[d] = ckey[3 * o + d] >> 1;
+        ind += (1 << d) * (ckey[3 * o + d] & 1);
     }
     const int code = resolve_oct(coarse, kf[0], kf[1], kf[2]);
     father[2 * i] = code > 0 ? code : 0;
@@ -417,6 +355,7 @@ static ResolveCtx make_resolve(int ilevel, const int32_t box_min[3], const int32
     c.koff = s.key_off[ilevel - 1];
     c.nx = s.ckey_max[ilevel - 1];
     c.ndim = s.ndim;
+    c.ngridmax = s.ngridmax;
     for (int d = 0; d < 3; d++) {
         c.boxmin[d] = d < s.ndim ? box_min[d] : 0;
         c.boxmax[d] = d < s.ndim ? box_max[d] : 1;
@@ -465,6 +404,7 @@ int rh_finalize(void)
 {
     State &s = st();
     if (!s.inited) return 0;
+    drop_level_graph(s);
     // no device synchronisation: released blocks go to the allocation cache
     // and are reused only by work ordered after this context's on the stream
     free_level_tables();
@@ -473,6 +413,11 @@ int rh_finalize(void)
     dfree(s.ckey);
     dfree(s.lev);
     dfree(s.refined);
+    dfree(s.flag1);
+    s.flag1 = nullptr;
+    dfree(s.racc_det);
+    s.racc_det = nullptr;
+    s.racc_det_cap = 0;
     dfree(s.hkeys);
     dfree(s.hvals);
     dfree(s.ghost_parent);
@@ -503,12 +448,18 @@ int rh_alloc_amr(int ngridmax, int ncachemax, int nvar, int twotondim, int hash_
         return fail(RH_ERR_ARG, "ngridmax+ncachemax must be in [1, 2^29]");
     if (hash_size < 1) return fail(RH_ERR_ARG, "hash_size must be positive");
     (void)hipStreamSynchronize(s.stream);
+    drop_level_graph(s);
     free_level_tables();
     dfree(s.uold);
     dfree(s.unew);
     dfree(s.ckey);
     dfree(s.lev);
     dfree(s.refined);
+    dfree(s.flag1);
+    s.flag1 = nullptr;
+    dfree(s.racc_det);
+    s.racc_det = nullptr;
+    s.racc_det_cap = 0;
     dfree(s.hkeys);
     dfree(s.hvals);
     s.ndim = twotondim == 8 ? 3 : 1;
@@ -687,6 +638,12 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
     (void)ckey_max_l;
     (void)key_off_l;
     if (!box_min || !box_max || !periodic) return fail(RH_ERR_ARG, "null box arrays");
+    drop_level_graph(s);  // its tables are rebuilt
+    if (s.ndim == 3 && !periodic[0] + !periodic[1] + !periodic[2] > 1)
+        // stencil octs beyond an edge or a corner of the box: the reference
+        // builds them from whichever neighbouring boundary oct its cache holds
+        // at that moment (amr/nbors_utils.f90:580-640), so no result is defined
+        return fail(RH_ERR_ARG, "3D boxes with more than one non-periodic dimension are not supported");
     int ilevel = 0;
     if (int r = level_of_range(head_idx, num_octs, ilevel)) return r;
     if (num_octs == 0) return 0;
@@ -735,7 +692,7 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
             RH_HIP_CHECK(hipGetLastError());
             RH_HIP_CHECK(hipMemcpyAsync(hc, s.d_counter, 4 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
             RH_HIP_CHECK(hipStreamSynchronize(s.stream));
-            if (hc[0] > 0) {
+            if (hc[0] > 0) {  // (the edges of a 1D box count here and make no ghost)
                 const int base = s.nghost_total;
                 RH_HIP_CHECK(hipMemsetAsync(s.d_counter, 0, 4 * sizeof(int), s.stream));
                 hipLaunchKernelGGL(k_make_ghosts, dim3(nblk), dim3(256), 0, s.stream, s.ckey, ln.nbor, head_idx,
@@ -749,47 +706,49 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
                     return fail(RH_ERR_MESH, "ncachemax too small for the ghost octs of the level (or hash full)");
                 }
                 const int nnew = hc[0];
-                RH_HIP_CHECK(hipMemsetAsync(s.d_counter, 0, 4 * sizeof(int), s.stream));
-                hipLaunchKernelGGL(k_ghost_sources, dim3((nnew + 127) / 128), dim3(128), 0, s.stream, s.ghost_key,
-                                   base, nnew, c, cc, s.ghost_parent, s.ghost_src, s.ghost_srckey, s.d_counter);
-                RH_HIP_CHECK(hipGetLastError());
-                RH_HIP_CHECK(hipMemcpyAsync(hc, s.d_counter, 4 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
-                RH_HIP_CHECK(hipStreamSynchronize(s.stream));
-                ln.ghost_first = s.ngridmax + 1 + base;
-                ln.nghost = nnew;
-                s.nghost_total += nnew;
-                {
-                    // the level's ghosts in parent-cell order: the interpolation
-                    // and the reflux fold walk them in this order, so that the
-                    // ghosts of one wave (and of one XCD) share parent octs
-                    std::vector<int32_t> par((size_t)nnew * 2), ord((size_t)nnew);
-                    RH_HIP_CHECK(hipMemcpyAsync(par.data(), s.ghost_parent + 2 * (size_t)base,
-                                                par.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream));
+                if (nnew > 0) {
+                    RH_HIP_CHECK(hipMemsetAsync(s.d_counter, 0, 4 * sizeof(int), s.stream));
+                    hipLaunchKernelGGL(k_ghost_sources, dim3((nnew + 127) / 128), dim3(128), 0, s.stream, s.ghost_key,
+                                       base, nnew, c, cc, s.ghost_parent, s.ghost_src, s.ghost_srckey, s.d_counter);
+                    RH_HIP_CHECK(hipGetLastError());
+                    RH_HIP_CHECK(hipMemcpyAsync(hc, s.d_counter, 4 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
                     RH_HIP_CHECK(hipStreamSynchronize(s.stream));
-                    for (int g = 0; g < nnew; g++) ord[g] = g;
-                    std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
-                        const int64_t kx = (int64_t)par[2 * x] * 8 + par[2 * x + 1];
-                        const int64_t ky = (int64_t)par[2 * y] * 8 + par[2 * y + 1];
-                        return kx < ky;
-                    });
-                    for (int g = 0; g < nnew; g++) ord[g] += base;
-                    RH_HIP_CHECK(hipMemcpyAsync(s.ghost_order + base, ord.data(), ord.size() * sizeof(int32_t),
-                                                hipMemcpyHostToDevice, s.stream));
-                    RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+                    ln.ghost_first = s.ngridmax + 1 + base;
+                    ln.nghost = nnew;
+                    s.nghost_total += nnew;
+                    {
+                        // the level's ghosts in parent-cell order: the interpolation
+                        // and the reflux fold walk them in this order, so that the
+                        // ghosts of one wave (and of one XCD) share parent octs
+                        std::vector<int32_t> par((size_t)nnew * 2), ord((size_t)nnew);
+                        RH_HIP_CHECK(hipMemcpyAsync(par.data(), s.ghost_parent + 2 * (size_t)base,
+                                                    par.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream));
+                        RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+                        for (int g = 0; g < nnew; g++) ord[g] = g;
+                        std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
+                            const int64_t kx = (int64_t)par[2 * x] * 8 + par[2 * x + 1];
+                            const int64_t ky = (int64_t)par[2 * y] * 8 + par[2 * y + 1];
+                            return kx < ky;
+                        });
+                        for (int g = 0; g < nnew; g++) ord[g] += base;
+                        RH_HIP_CHECK(hipMemcpyAsync(s.ghost_order + base, ord.data(), ord.size() * sizeof(int32_t),
+                                                    hipMemcpyHostToDevice, s.stream));
+                        RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+                    }
+                    for (int g = 0; g < nnew; g++) s.h_lev[s.ngridmax + base + g] = ilevel;
+                    if (hc[2] & 1) {
+                        s.nbors[ilevel] = ln;
+                        return fail(RH_ERR_MESH, "GODUNOV: parent_cell should exist");
+                    }
+                    if (hc[2] & 2) {
+                        s.nbors[ilevel] = ln;
+                        return fail(RH_ERR_MESH, "GODUNOV: parent neighbors should exist");
+                    }
+                    // the ghosts now resolve through the hash
+                    hipLaunchKernelGGL(k_build_nbor, dim3(nblk), dim3(256), 0, s.stream, s.ckey, head_idx, num_octs, c,
+                                       nslot, ln.nbor);
+                    RH_HIP_CHECK(hipGetLastError());
                 }
-                for (int g = 0; g < nnew; g++) s.h_lev[s.ngridmax + base + g] = ilevel;
-                if (hc[2] & 1) {
-                    s.nbors[ilevel] = ln;
-                    return fail(RH_ERR_MESH, "GODUNOV: parent_cell should exist");
-                }
-                if (hc[2] & 2) {
-                    s.nbors[ilevel] = ln;
-                    return fail(RH_ERR_MESH, "GODUNOV: parent neighbors should exist");
-                }
-                // the ghosts now resolve through the hash
-                hipLaunchKernelGGL(k_build_nbor, dim3(nblk), dim3(256), 0, s.stream, s.ckey, head_idx, num_octs, c,
-                                   nslot, ln.nbor);
-                RH_HIP_CHECK(hipGetLastError());
             }
         }
         s.nbors[ilevel] = ln;
@@ -809,6 +768,7 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
         lb.bx = s.brick_bx;
         lb.by = s.brick_by;
         lb.bz = s.brick_bz;
+        lb.bnd = !(c.periodic[0] && c.periodic[1] && c.periodic[2]);
         // enumerate the bricks that hold at least one oct of the level
         std::vector<int64_t> keys;
         const int64_t nbx = (c.boxmax[0] - c.boxmin[0] + lb.bx - 1) / lb.bx;
@@ -988,6 +948,21 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         RH_HIP_CHECK(launch_refresh_ghosts(ga, s.stream));
     }
     const int reflux = ilevel > levelmin ? 1 : 0;
+    // deterministic refluxes: per-face slots of the level's ghosts, folded into
+    // the parent cells in slot order after the launch
+    double *racc = nullptr;
+    if (reflux && s.deterministic && lnb.nghost > 0) {
+        if (s.racc_det_cap < s.nghost_total) {
+            dfree(s.racc_det);
+            s.racc_det = nullptr;
+            s.racc_det_cap = 0;
+            const long n = 5l * kRefluxSlotsPerGhost * s.nghost_total;
+            RH_HIP_CHECK(dmalloc((void **)&s.racc_det, (size_t)n * sizeof(double)));
+            RH_HIP_CHECK(launch_zero(s.racc_det, n, s.stream));  // each fold leaves it zeroed
+            s.racc_det_cap = s.nghost_total;
+        }
+        racc = s.racc_det;
+    }
     if (s.ndim == 3) {
         auto it = s.bricks.find(ilevel);
         if (it == s.bricks.end() || it->second.head != head_idx || it->second.noct != num_subgrids)
@@ -1002,6 +977,7 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         a.org = lb.org;
         a.zr = lb.zr;
         a.any_ref = lb.any_ref;
+        a.bnd = lb.bnd;
         a.nbricks = lb.nbricks;
         a.head = head_idx;
         a.noct = num_subgrids;
@@ -1030,7 +1006,8 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         a.ghost_parent = s.ghost_parent;
         a.courant = 0;
         a.cpart = nullptr;
-        a.racc = nullptr;
+        a.racc = racc;
+        a.rdet = racc != nullptr;
         RH_HIP_CHECK(launch_godunov3d(a, lb.bx, lb.by, lb.bz, s.precision, s.stream));
     } else {
         const LevelNbor &ln = lnb;
@@ -1062,7 +1039,75 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         a.reflux = reflux;
         a.ngridmax = s.ngridmax;
         a.ghost_parent = s.ghost_parent;
+        a.racc = racc;
+        a.rdet = racc != nullptr;
         RH_HIP_CHECK(launch_godunov1d(a, s.precision, s.stream));
+    }
+    if (racc)
+        RH_HIP_CHECK(launch_reflux_fold(s.unew, racc, s.ghost_parent, s.ghost_order + (lnb.ghost_first - s.ngridmax - 1),
+                                        lnb.nghost, s.ttd, s.stream, kRefluxSlotsPerGhost));
+    return 0;
+}
+
+int rh_get_level_table(int ilevel, int which, int32_t *out, int64_t capacity, int64_t *len)
+{
+    if (int r = ensure_mesh()) return r;
+    State &s = st();
+    if (!len) return fail(RH_ERR_ARG, "rh_get_level_table: null len");
+    *len = 0;
+    auto nb = s.nbors.find(ilevel);
+    if (nb == s.nbors.end()) return fail(RH_ERR_STATE, "rh_get_level_table: rh_build_nbor was not called for this level");
+    const LevelNbor &ln = nb->second;
+    const int nslot = s.ndim == 3 ? 27 : 3;
+    const int32_t *src = nullptr;
+    int64_t n = 0;
+    std::vector<int32_t> host;
+    switch (which) {
+    case 0: src = ln.nbor; n = (int64_t)ln.noct * nslot; break;
+    case 1: src = ln.father; n = ln.father ? (int64_t)ln.noct * 2 : 0; break;
+    case 2:
+    case 3: {
+        auto bk = s.bricks.find(ilevel);
+        if (bk == s.bricks.end()) return fail(RH_ERR_STATE, "rh_get_level_table: the level has no bricks (1D)");
+        const LevelBricks &lb = bk->second;
+        const int64_t nw = (int64_t)(lb.bx + 2) * (lb.by + 2) * (lb.bz + 2);
+        src = which == 2 ? lb.org : lb.win;
+        n = which == 2 ? (int64_t)lb.nbricks * 3 : (int64_t)lb.nbricks * nw;
+        break;
+    }
+    case 4:
+    case 5: {
+        const int g0 = ln.ghost_first > 0 ? ln.ghost_first - s.ngridmax - 1 : 0;
+        n = (int64_t)ln.nghost * (which == 4 ? 4 : 2);
+        if (capacity < n) break;
+        if (n > 0 && !out) return fail(RH_ERR_ARG, "rh_get_level_table: null out");
+        host.resize((size_t)ln.nghost * 3);
+        if (ln.nghost > 0) {
+            RH_HIP_CHECK(hipMemcpyAsync(host.data(), which == 4 ? s.ghost_key + 3 * (size_t)g0 : s.ghost_parent + 2 * (size_t)g0,
+                                        (size_t)ln.nghost * (which == 4 ? 3 : 2) * sizeof(int32_t),
+                                        hipMemcpyDeviceToHost, s.stream));
+            RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+        }
+        for (int g = 0; g < ln.nghost; g++) {
+            if (which == 4) {
+                out[4 * g] = ln.ghost_first + g;
+                for (int d = 0; d < 3; d++) out[4 * g + 1 + d] = host[3 * (size_t)g + d];
+            } else {
+                out[2 * g] = host[2 * (size_t)g];
+                out[2 * g + 1] = host[2 * (size_t)g + 1];
+            }
+        }
+        *len = n;
+        return 0;
+    }
+    default: return fail(RH_ERR_ARG, "rh_get_level_table: which must be 0..5");
+    }
+    *len = n;
+    if (capacity < n) return fail(RH_ERR_ARG, "rh_get_level_table: capacity too small");
+    if (n > 0) {
+        if (!out) return fail(RH_ERR_ARG, "rh_get_level_table: null out");
+        RH_HIP_CHECK(hipMemcpyAsync(out, src, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream));
+        RH_HIP_CHECK(hipStreamSynchronize(s.stream));
     }
     return 0;
 }

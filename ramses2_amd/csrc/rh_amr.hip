@@ -80,7 +80,9 @@ hipError_t enqueue_amr_step(State &s, int parity, int precision)
         if ((e = launch_godunov3d_wl(w, gr.bx, gr.by, gr.bz, m.slope, m.riemann, precision, s.stream)) != hipSuccess)
             return e;
     }
-    if ((e = launch_reflux_fold(unext, m.racc, s.ghost_parent, s.ghost_order, s.nghost_total, 8, s.stream)) != hipSuccess) return e;
+    if ((e = launch_reflux_fold(unext, m.racc, s.ghost_parent, s.ghost_order, s.nghost_total, 8, s.stream,
+                                m.nslot)) != hipSuccess)
+        return e;
     for (int i = (int)m.levels.size() - 1; i >= 1; i--) {
         const int lf = m.levels[i];
         if (m.levels[i - 1] != lf - 1) continue;
@@ -166,10 +168,17 @@ int rh_amr_prepare(int levelmin, int levelmax, const int32_t *head, const int32_
     m.racc = nullptr;
     m.d_descs = nullptr;
     m.d_items = nullptr;
-    RH_HIP_CHECK(dmalloc((void **)&m.racc, 5 * sizeof(double) * (size_t)std::max(s.nghost_total, 1)));
+    // one accumulator per ghost oct, or one per face of each ghost oct in
+    // deterministic mode (rh_set_deterministic)
+    m.nslot = s.deterministic ? kRefluxSlotsPerGhost : 1;
+    const long nacc = 5l * m.nslot * std::max(s.nghost_total, 1);
+    RH_HIP_CHECK(dmalloc((void **)&m.racc, sizeof(double) * (size_t)nacc));
     // zeroed once: each step's fold leaves it zeroed again
-    RH_HIP_CHECK(launch_zero(m.racc, 5l * std::max(s.nghost_total, 1), s.stream));
-    for (auto &a : descs) a.racc = m.racc;
+    RH_HIP_CHECK(launch_zero(m.racc, nacc, s.stream));
+    for (auto &a : descs) {
+        a.racc = m.racc;
+        a.rdet = s.deterministic;
+    }
     m.ndesc = (int)descs.size();
     std::vector<GodArgs> both(2 * descs.size());
     for (size_t i = 0; i < descs.size(); i++) {

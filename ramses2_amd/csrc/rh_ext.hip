@@ -3,6 +3,8 @@
 // single-level mesh with the Courant dt kept on the device (optionally replayed
 // as a captured hipGraph).
 #include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
 
 #include <string>
 #include <vector>
@@ -58,6 +60,7 @@ int fill_args3(int head_idx, int num_octs, int ilevel, double dx, double gamma, 
     a.org = lb.org;
     a.zr = lb.zr;
     a.any_ref = lb.any_ref;
+    a.bnd = lb.bnd;
     a.nbricks = lb.nbricks;
     a.head = head_idx;
     a.noct = num_octs;
@@ -131,11 +134,27 @@ int fill_args1(int head_idx, int num_octs, int ilevel, double dx, double gamma, 
     return 0;
 }
 
+static void drop_graph_entry(State &s, int i)
+{
+    if (s.level_exec[i]) (void)hipGraphExecDestroy(s.level_exec[i]);
+    if (s.level_graph[i]) (void)hipGraphDestroy(s.level_graph[i]);
+    s.level_exec[i] = nullptr;
+    s.level_graph[i] = nullptr;
+    s.graph_key[i].clear();
+}
+
+void drop_level_graph(State &s)
+{
+    drop_graph_entry(s, 0);
+    drop_graph_entry(s, 1);
+}
+
 // Grow the partial-sum buffer to n entries of 4 doubles.
 int ensure_partials(int n)
 {
     State &s = st();
     if (n <= s.red_cap) return 0;
+    drop_level_graph(s);
     dfree(s.red_part);
     s.red_part = nullptr;
     RH_HIP_CHECK(dmalloc((void **)&s.red_part, (size_t)n * 4 * sizeof(double)));
@@ -173,9 +192,10 @@ hipError_t enqueue_step(int head_idx, int num_octs, double dx, double gamma, dou
         a3->ubase = s.uold;
         a3->courant = 1;
         a3->cpart = s.red_part;
-        hipError_t e = launch_godunov3d(*a3, lb->bx, lb->by, lb->bz, s.precision, s.stream, e0, e1);
+        int nparts = 0;
+        hipError_t e = launch_godunov3d(*a3, lb->bx, lb->by, lb->bz, s.precision, s.stream, e0, e1, &nparts);
         if (e != hipSuccess) return e;
-        return launch_courant_reduce(s.red_part, lb->nbricks, dx / __builtin_sqrt(smallc2), s.red_out, s.dt_dev,
+        return launch_courant_reduce(s.red_part, nparts, dx / __builtin_sqrt(smallc2), s.red_out, s.dt_dev,
                                      s.t_dev, s.stream);
     }
     if (dt_fixed <= 0.0) {
@@ -233,6 +253,13 @@ int rh_set_brick_shape(int bx, int by, int bz)
     return 0;
 }
 
+int rh_set_deterministic(int on)
+{
+    if (on != 0 && on != 1) return efail(RH_ERR_ARG, "rh_set_deterministic: 0 or 1");
+    st().deterministic = on;
+    return 0;
+}
+
 int rh_get_brick_shape(int32_t *bx, int32_t *by, int32_t *bz)
 {
     if (bx) *bx = st().brick_bx;
@@ -283,36 +310,89 @@ int rh_level_run(int head_idx, int num_octs, int ilevel, double dx, double gamma
     if (s.ndim == 3 && dt_fixed <= 0.0) {
         // dt of the first step from a standalone Courant pass; each step's
         // kernel then produces the dt of the next one
-        if (int r2 = ensure_partials(lb->nbricks)) return r2;
+        if (int r2 = ensure_partials(2 * lb->nbricks + 8)) return r2;  // bricks and split halves
         const double gz[3] = {0, 0, 0};
         RH_HIP_CHECK(launch_courant(s.uold, s.refined, head_idx, num_octs, s.ndim, dx, gamma, smallr,
                                     __builtin_sqrt(smallc2), courant_factor, g ? g : gz, s.red_part, s.red_cap,
                                     s.red_out, s.dt_dev, s.stream, nullptr));
     }
     if (use_graph && nsteps >= 2) {
-        // capture the pair of steps uold->unew, unew->uold as one graph
-        hipGraph_t graph;
-        hipGraphExec_t exec;
-        RH_HIP_CHECK(hipStreamSynchronize(s.stream));
-        RH_HIP_CHECK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < 2; k++) {
-            hipError_t e = enqueue_step(head_idx, num_octs, dx, gamma, smallr, smallc2, courant_factor, g, dt_fixed,
-                                        s.ndim == 3 ? &a3 : nullptr, s.ndim == 3 ? nullptr : &a1, lb);
-            std::swap(s.uold, s.unew);
+        // the pair of steps uold->unew, unew->uold as one graph, captured once
+        // for each order of the two arrays and kept for later calls with the
+        // same arguments, arrays and tables
+        auto make_key = [&]() {
+            std::vector<uint64_t> key;
+            auto put = [&](double v) {
+                uint64_t b;
+                memcpy(&b, &v, sizeof b);
+                key.push_back(b);
+            };
+            for (double v : {(double)head_idx, (double)num_octs, (double)ilevel, dx, gamma, smallr, smallc2,
+                             courant_factor, (double)slope, (double)riemann, dt_fixed, (double)s.precision,
+                             (double)courant_fused()})
+                put(v);
+            for (int d = 0; d < 3; d++) put(g ? g[d] : 0.0);
+            for (int d = 0; d < 3; d++) put(s.btype[d][0] * 4.0 + s.btype[d][1]);
+            if (lb)
+                for (int v : {lb->nbricks, lb->bx, lb->by, lb->bz, lb->any_ref, lb->bnd, lb->head, lb->noct}) put(v);
+            for (const void *q : {(const void *)s.uold, (const void *)s.unew, (const void *)s.refined,
+                                  (const void *)s.red_part, (const void *)s.red_out, (const void *)s.dt_dev,
+                                  (const void *)s.t_dev, lb ? (const void *)lb->win : nullptr,
+                                  lb ? (const void *)lb->org : nullptr, lb ? (const void *)lb->zr : nullptr})
+                key.push_back((uint64_t)(uintptr_t)q);
+            return key;
+        };
+        // capture the step pair in the arrays' current order into entry ge
+        auto capture = [&](int ge) -> int {
+            drop_graph_entry(s, ge);
+            const std::vector<uint64_t> key = make_key();
+            hipGraph_t graph = nullptr;
+            RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+            RH_HIP_CHECK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < 2; k++) {
+                hipError_t e = enqueue_step(head_idx, num_octs, dx, gamma, smallr, smallc2, courant_factor, g,
+                                            dt_fixed, s.ndim == 3 ? &a3 : nullptr, s.ndim == 3 ? nullptr : &a1, lb);
+                if (e != hipSuccess) {
+                    // the arrays stay where they were (a swap only follows a captured step)
+                    if (k == 1) std::swap(s.uold, s.unew);
+                    hipGraph_t g2 = nullptr;
+                    (void)hipStreamEndCapture(s.stream, &g2);
+                    if (g2) (void)hipGraphDestroy(g2);
+                    set_error(hipGetErrorString(e));
+                    return RH_ERR_HIP;
+                }
+                std::swap(s.uold, s.unew);
+            }
+            RH_HIP_CHECK(hipStreamEndCapture(s.stream, &graph));
+            hipGraphExec_t exec = nullptr;
+            hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             if (e != hipSuccess) {
-                hipGraph_t g2;
-                (void)hipStreamEndCapture(s.stream, &g2);
-                set_error(hipGetErrorString(e));
+                (void)hipGraphDestroy(graph);
+                set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
                 return RH_ERR_HIP;
             }
+            s.level_graph[ge] = graph;
+            s.level_exec[ge] = exec;
+            s.graph_key[ge] = key;
+            return 0;
+        };
+        const std::vector<uint64_t> key = make_key();
+        int ge = -1;
+        for (int i = 0; i < 2; i++)
+            if (s.level_exec[i] && key == s.graph_key[i]) ge = i;
+        if (ge < 0) {
+            // both orders at once, so that a later call after an odd number
+            // of steps replays without capturing
+            if (int r = capture(0)) return r;
+            std::swap(s.uold, s.unew);
+            const int r1 = capture(1);
+            std::swap(s.uold, s.unew);
+            if (r1) return r1;
+            ge = 0;
         }
-        RH_HIP_CHECK(hipStreamEndCapture(s.stream, &graph));
-        RH_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         const int npairs = nsteps / 2;
-        for (int i = 0; i < npairs; i++) RH_HIP_CHECK(hipGraphLaunch(exec, s.stream));
+        for (int i = 0; i < npairs; i++) RH_HIP_CHECK(hipGraphLaunch(s.level_exec[ge], s.stream));
         RH_HIP_CHECK(hipStreamSynchronize(s.stream));
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
         // after an even number of swaps the roles are back where they started;
         // the state of the last step is in uold (the captured pair ended on a swap)
         if (nsteps % 2) {

@@ -32,12 +32,18 @@ struct GodArgs {
     double g[3];
     int slope, riemann;
     int xcd_remap;            // remap block ids so each XCD gets contiguous bricks
+    // set by launch_godunov3d: bricks at the end of each XCD's range that run
+    // as two half-brick work items (0 for levels with refluxes)
+    int nsplit = 0;
     // coarse-fine refluxing (levels above levelmin): octs > ngridmax are
     // interpolated ghosts whose parent cell is ghost_parent[2*(v-ngridmax-1)+{0,1}]
     int reflux;
     // 0 when no cell of the level is refined: the kernel then skips the
     // refined flags (their loads, the zeroing of refined faces)
     int any_ref;
+    // 0 when the level is periodic in every dimension: its brick windows hold
+    // no boundary ghosts, and the gather skips their synthesis
+    int bnd = 1;
     int ngridmax;
     const int32_t *ghost_parent;
     // fused courant_fine of the updated state: per-brick partials
@@ -48,6 +54,9 @@ struct GodArgs {
     // per ghost oct, i.e. per parent cell) instead of into the parent cell of
     // unew; a later fold adds them (several levels in one launch)
     double *racc;
+    // 1: racc holds kRefluxSlotsPerGhost slots per ghost, each written by one
+    // face (deterministic reflux); 0: one accumulator per ghost (atomics)
+    int rdet = 0;
 };
 
 // Work list of bricks over several descriptors (levels / jobs): item i is
@@ -79,12 +88,16 @@ struct God1Args {
     int reflux;
     int ngridmax;
     const int32_t *ghost_parent;
+    double *racc = nullptr;   // as GodArgs::racc / rdet
+    int rdet = 0;
 };
 
 // Launch the 3D brick kernel for brick shape (bx,by,bz) in precision prec
 // (0 = FP64, 1 = FP32 reconstruction/fluxes with FP64 accumulation).
+// *nparts (if non-null) receives the number of Courant partials the launch
+// writes (bricks plus split halves), at most 2 nbricks + 8.
 hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, hipStream_t s,
-                            hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                            hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, int *nparts = nullptr);
 hipError_t launch_godunov1d(const God1Args &a, int prec, hipStream_t s);
 // Work-list launch: items[i] = (descriptor index, brick index), one workgroup per item.
 hipError_t launch_godunov3d_wl(const WorkList &w, int bx, int by, int bz, int slope, int riemann, int prec,
@@ -113,9 +126,13 @@ hipError_t launch_courant_ml(const double *uold, const uint8_t *refined, const i
                              double *dt_out, hipStream_t s, double *t_acc);
 // racc[0 .. 5*n) = 0 (a kernel: safe inside captured graphs)
 hipError_t launch_zero(double *p, long n, hipStream_t s);
-// unew(parent cell of ghost g) += racc[5*g .. 5*g+4] for the ghosts g in [0, nghost)
+// unew(parent cell of ghost g) += the nslot accumulators of g (racc[5*nslot*g
+// ...], added in slot order) for the ghosts order[0 .. nghost), which are
+// cleared; nslot = 1 (atomic refluxes) or kRefluxSlots (deterministic mode)
 hipError_t launch_reflux_fold(double *unew, double *racc, const int32_t *ghost_parent, const int32_t *order,
-                              int nghost, int ttd, hipStream_t s);
+                              int nghost, int ttd, hipStream_t s, int nslot = 1);
+// Slots per ghost oct of the deterministic reflux: 6 faces x 4 cells
+constexpr int kRefluxSlotsPerGhost = 24;
 hipError_t launch_courant_reduce_batch(const CourantJob *jobs, int njobs, hipStream_t s);
 hipError_t launch_courant_reduce(const double *partials, int nparts, double dtmax, double *out, double *dt_inout,
                                  double *t_acc, hipStream_t s);

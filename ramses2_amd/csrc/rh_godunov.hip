@@ -20,6 +20,10 @@
 // No MFMA: the work is divides, square roots and compares, not a contraction.
 #include <hip/hip_ext.h>
 
+#include <map>
+#include <type_traits>
+
+#include "rh_gather.h"
 #include "rh_godunov.h"
 
 namespace rh {
@@ -234,85 +238,12 @@ __device__ __forceinline__ void gather_cell_m(const double *__restrict__ uold, c
     rf = rflag_src_valid ? refined[(size_t)(ref - 1) * 8 + rs[0] + 2 * rs[1] + 4 * rs[2]] : 0;
 }
 
-__device__ __forceinline__ int reflexive_mask(const int btype[3][2])
-{
-    int refl = 0;
-#pragma unroll
-    for (int d = 0; d < 3; d++)
-#pragma unroll
-        for (int sd = 0; sd < 2; sd++) refl |= (btype[d][sd] == 1) << (2 * d + sd);
-    return refl;
-}
-
 __device__ __forceinline__ void gather_cell(const double *__restrict__ uold, const uint8_t *__restrict__ refined,
                                             int v, const int okey[3], int ndim, const int boxmin[3],
                                             const int boxmax[3], const int btype[3][2], int c[3], double u[5],
                                             uint8_t &rf)
 {
     gather_cell_m(uold, refined, v, okey, ndim, boxmin, boxmax, reflexive_mask(btype), c, u, rf);
-}
-
-// gather_cell_m for 3D split into the address computation and the loads, so
-// that the loads of the next plane can be issued a plane ahead: the cell's
-// five values start at p (stride 8; null: no oct, zero state), its refined
-// flag is *rf (null: 0), and momentum component flip (0..2, or -1) is negated
-// when the values are used (reflexive boundary ghost).
-struct CellRef {
-    const double *p;
-    const uint8_t *rf;
-    int flip;
-};
-
-__device__ __forceinline__ CellRef cell_ref3(const double *uold, const uint8_t *refined, int v, const int okey[3],
-                                             const int boxmin[3], const int boxmax[3], int refl, const int c[3])
-{
-    CellRef r;
-    r.flip = -1;
-    if (v > 0) {
-        const int ind = c[0] + 2 * c[1] + 4 * c[2];
-        r.p = uold + (size_t)(v - 1) * 40 + ind;
-        r.rf = refined + (size_t)(v - 1) * 8 + ind;
-        return r;
-    }
-    if (v == 0) {
-        r.p = nullptr;
-        r.rf = nullptr;
-        return r;
-    }
-    int dir = 0, side = 0;
-    for (int d = 2; d >= 0; d--) {
-        if (okey[d] < boxmin[d]) { dir = d; side = 0; }
-        else if (okey[d] >= boxmax[d]) { dir = d; side = 1; }
-    }
-    const int type = (refl >> (2 * dir + side)) & 1 ? 1 : 2;
-    int s3[3] = {c[0], c[1], c[2]}, rs[3] = {c[0], c[1], c[2]};
-    const bool rvalid = side == 0 ? c[dir] == 1 : c[dir] == 0;
-    rs[dir] = side == 0 ? 0 : 1;
-    s3[dir] = type == 1 ? 1 - c[dir] : (side == 0 ? 0 : 1);
-    const int ref = -v;
-    r.p = uold + (size_t)(ref - 1) * 40 + (s3[0] + 2 * s3[1] + 4 * s3[2]);
-    r.rf = rvalid ? refined + (size_t)(ref - 1) * 8 + rs[0] + 2 * rs[1] + 4 * rs[2] : nullptr;
-    r.flip = type == 1 ? dir : -1;
-    return r;
-}
-
-__device__ __forceinline__ void cell_load3(const CellRef &r, double u[5], uint8_t &rf)
-{
-    if (r.p) {
-#pragma unroll
-        for (int n = 0; n < 5; n++) u[n] = r.p[8 * n];
-    } else {
-#pragma unroll
-        for (int n = 0; n < 5; n++) u[n] = 0.0;
-    }
-    rf = r.rf ? *r.rf : (uint8_t)0;
-}
-
-__device__ __forceinline__ void cell_flip(double u[5], int flip)
-{
-    u[1] = flip == 0 ? -u[1] : u[1];
-    u[2] = flip == 1 ? -u[2] : u[2];
-    u[3] = flip == 2 ? -u[3] : u[3];
 }
 
 template <typename T>
@@ -375,14 +306,29 @@ __device__ __forceinline__ void slope27(const T (*sq)[5][NP], int sm, int s0, in
 // Coarse-fine refluxing, godfine1 (hydro/godunov_fine.f90:894-1052): a face
 // between a cell of the level and an interpolated ghost cell sends -flux/2^ndim
 // (ghost on the low side) or +flux/2^ndim (ghost on the high side) to the
-// ghost's parent cell.  FP64 atomics: the order of the contributions to one
-// parent cell is not fixed (last-bit differences from the serial reference).
+// ghost's parent cell.  Three ways:
+// - racc null: FP64 atomics into the parent cell of unew;
+// - racc, rdet 0: FP64 atomics into the ghost's accumulator racc[5 g];
+// either way the order of the contributions to one parent cell is not fixed
+// (last-bit differences from run to run and from the serial reference);
+// - racc, rdet 1 (deterministic mode): a plain store into the face's own slot
+//   racc[5 (kReluxSlots g + slot)], slot = 4 side + sub (side: the ghost oct's
+//   face, -x +x -y +y -z +z; sub: which of its 2^(ndim-1) cells), one writer
+//   per slot; k_reflux_fold then adds the slots in slot order.
+constexpr int kRefluxSlots = kRefluxSlotsPerGhost;
 __device__ __forceinline__ void reflux_add(double *unew, const int32_t *ghost_parent, int ngridmax, int vghost,
-                                           double sign, const double F[5], int ttd, double *racc = nullptr)
+                                           double sign, const double F[5], int ttd, double *racc = nullptr,
+                                           int rdet = 0, int slot = 0)
 {
     const int gi = vghost - ngridmax - 1;
     if (racc) {
         const double w = 1.0 / (double)ttd;
+        if (rdet) {
+#pragma unroll
+            for (int n = 0; n < 5; n++)
+                racc[5 * ((size_t)gi * kRefluxSlots + slot) + n] = sign * (F[n] * w);
+            return;
+        }
 #pragma unroll
         for (int n = 0; n < 5; n++) atomicAdd(racc + 5 * (size_t)gi + n, sign * (F[n] * w));
         return;
@@ -399,6 +345,15 @@ __device__ __forceinline__ void reflux_add(double *unew, const int32_t *ghost_pa
 // (FP32: the per-level kernel only; it runs at two waves per SIMD with them,
 // measured 4% faster at 256^3 than at three without, while the work-list
 // kernel keeps four waves per SIMD).
+// The z march unrolled by two planes.
+#ifndef RH_MARCH_UNROLL2
+#define RH_MARCH_UNROLL2 0
+#endif
+// Levels periodic in every dimension gather through a path without the
+// boundary-ghost synthesis (a uniform branch per launch).
+#ifndef RH_FAST_FETCH
+#define RH_FAST_FETCH 1
+#endif
 #ifndef RH_PREFETCH_F64
 #define RH_PREFETCH_F64 1
 #endif
@@ -406,6 +361,11 @@ __device__ __forceinline__ void reflux_add(double *unew, const int32_t *ghost_pa
 #define RH_PREFETCH_F32 1
 #endif
 
+// unew stored one oct layer (two z-planes) at a time, in whole 64-byte lines
+// (FP64 kernels: the FP32 kernel of three waves per SIMD would spill).
+#ifndef RH_PAIR_WRITES
+#define RH_PAIR_WRITES 1
+#endif
 // LDS of one brick workgroup.  NS primitive planes are kept: 2 when the slopes
 // need only face neighbours (the z neighbours of a column sit in registers),
 // 3 for the 27-point slope of slope_type 3.
@@ -422,6 +382,10 @@ template <int BX, int BY, int BZ, typename T, int NS, bool SEP> struct BrickSmem
     uint8_t sref[NS][NP];
     T sf[2][5][NT];
     T sff[SEP ? 2 : 1][5][SEP ? NT : 1];
+    // new state of the lower z-plane of the oct layer, per interior column
+    // (RH_PAIR_WRITES)
+    static constexpr bool PW = RH_PAIR_WRITES && sizeof(T) == 8;
+    double sun[PW ? 5 : 1][PW ? 4 * BX * BY : 1];
 };
 
 // The FP32 per-level kernel's Courant reduction in the update phase (as the
@@ -463,7 +427,8 @@ __device__ __forceinline__ int ring_slot3(int t) { return (t + 3) % 3; }
 // (a work-list descriptor): both are read-only for the kernel's lifetime, so
 // the compiler reloads their fields with scalar loads instead of holding them.
 template <int BX, int BY, int BZ, typename T, int RS, int NS, bool WL, typename A>
-__device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY, BZ, T, NS, sep_flux<BX, BY, BZ, T, NS>()> &sm)
+__device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY, BZ, T, NS, sep_flux<BX, BY, BZ, T, NS>()> &sm,
+                                           const int part = -1, const int pslot = -1)
 {
     constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
     constexpr int PX = 2 * BX + 4, PY = 2 * BY + 4, NP = PX * PY;
@@ -485,7 +450,16 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     // cell planes [tlo, thi] of the brick hold octs of the level; the march
     // covers them plus two halo planes on each side
     const int zr = a.zr ? a.zr[bid] : ((BZ - 1) << 16);
-    const int tlo = 2 * (zr & 0xffff), thi = 2 * (zr >> 16) + 1;
+    int zlo = zr & 0xffff, zhi = zr >> 16;
+    if (part >= 0) {
+        // half of the brick's z layers (the lower half holds the middle one)
+        const int mid = (zlo + zhi) >> 1;
+        if (part == 0)
+            zhi = mid;
+        else
+            zlo = mid + 1;
+    }
+    const int tlo = 2 * zlo, thi = 2 * zhi + 1;
 
     const double dt = a.dt_dev ? *a.dt_dev : a.dt;
     const HydroK<T> k = make_k<T>(dt, a.dx, a.gamma, a.smallr, a.smallc2, a.courant_factor, a.smax_cap, a.use_smax,
@@ -605,12 +579,23 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         }
         ring_i = (ry + 2) * PX + (rx + 2);
     }
+    const bool bnd = RH_FAST_FETCH ? a.bnd != 0 : true;
     auto fetch = [&](int i, int tz, double u[5], uint8_t &rf, int &fl) {
         const int px = i % PX - 2, py = i / PX - 2;
         const int ox = (px + 2) >> 1, oy = (py + 2) >> 1, oz = (tz + 2) >> 1;
         const int v = swin[(oz * WY + oy) * WX + ox];
-        const int okey[3] = {ox0 + ox - 1, oy0 + oy - 1, oz0 + oz - 1};
         const int c[3] = {px & 1, py & 1, tz & 1};
+        if (!bnd) {
+            // no boundary ghosts: an oct (v > 0) or none (v == 0)
+            const int ind = c[0] + 2 * c[1] + 4 * c[2];
+            CellRef r;
+            r.p = v > 0 ? a.uold + (size_t)(v - 1) * 40 + ind : nullptr;
+            r.rf = anyref && v > 0 ? a.refined + (size_t)(v - 1) * 8 + ind : nullptr;
+            cell_load3(r, u, rf);
+            fl = -1;
+            return;
+        }
+        const int okey[3] = {ox0 + ox - 1, oy0 + oy - 1, oz0 + oz - 1};
         CellRef r = cell_ref3(a.uold, a.refined, v, okey, boxmin, boxmax, refl, c);
         if (!anyref) r.rf = nullptr;
         cell_load3(r, u, rf);
@@ -632,8 +617,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     }
 
     // ---- P: primitives of plane t into ring slot st, own column to qcol_p ----
-    auto p_phase = [&](const int t, T qcol_p[5]) {
-        const int st = NS == 3 ? ring_slot3(t) : ring_slot2(t);
+    auto p_phase = [&](const int t, const int st, T qcol_p[5]) {
         if constexpr (!PF) {
             for (int i = tid; i < NP; i += NT) {
                 const int px = i % PX - 2, py = i / PX - 2;
@@ -654,7 +638,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             for (int n = 0; n < 5; n++) qcol_p[n] = sq[st][n][pc];
         } else {
             double q[5];
-            if (fl_o >= 0) cell_flip(u_o, fl_o);  // reflexive ghosts only: skipped by waves without any
+            if (bnd && fl_o >= 0) cell_flip(u_o, fl_o);  // reflexive ghosts only: skipped by waves without any
             prim_from_cons<sizeof(T) == 4>(u_o, q, a.smallr, smalle, a.gamma, gdt);
 #pragma unroll
             for (int n = 0; n < 5; n++) {
@@ -663,7 +647,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             }
             sref[st][pc] = rf_o;
             if (ring) {
-                if (fl_r >= 0) cell_flip(u_r, fl_r);
+                if (bnd && fl_r >= 0) cell_flip(u_r, fl_r);
                 prim_from_cons<sizeof(T) == 4>(u_r, q, a.smallr, smalle, a.gamma, gdt);
 #pragma unroll
                 for (int n = 0; n < 5; n++) sq[st][n][ring_i] = T(q[n]);
@@ -694,7 +678,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     // prefetch loads and the update's stores)
     for (int t = tlo - 2; t < tlo; ++t) {
         T qcol_p[5];
-        p_phase(t, qcol_p);
+        p_phase(t, NS == 3 ? ring_slot3(t) : ring_slot2(t), qcol_p);
         rotate(qcol_p);
     }
     // courant_fine of the state written (the dt of the next step), when asked
@@ -705,10 +689,11 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     const CourantK ck = make_courant_k(a.dx, 3, a.gamma, a.smallr, a.smallc2, a.courant_factor, gvec);
     const double gf0 = grav_factor0(ck);
     double c_mass = 0.0, c_ekin = 0.0, c_eint = 0.0, c_dt = a.dx / ck.smallc;
-    for (int t = tlo; t <= thi + 2; ++t) {
-        const int st = NS == 3 ? ring_slot3(t) : ring_slot2(t);
+    // one plane of the march; par = t & 1 (the ring slot of plane t when NS == 2)
+    auto march = [&](const int t, const int par) __attribute__((always_inline)) {
+        const int st = NS == 3 ? ring_slot3(t) : par;
         T qcol_p[5];
-        p_phase(t, qcol_p);
+        p_phase(t, st, qcol_p);
         // ---- T: slopes and predictor of plane tt = t-1 ----
         const int tt = t - 1;
         const bool plane_out = tt >= tlo && tt <= thi;
@@ -727,7 +712,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 for (int n = 0; n < 5; n++) ub[n] = pb[8 * n];
             }
         }
-        const int s0 = NS == 3 ? ring_slot3(t - 1) : ring_slot2(t - 1);
+        const int s0 = NS == 3 ? ring_slot3(t - 1) : par ^ 1;
         const int smm = NS == 3 ? ring_slot3(t - 2) : 0;
         T q0[5], dq[3][5];
 #pragma unroll
@@ -855,22 +840,31 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 const int vl = wv(cx - 1, cy, tt), vh = wv(cx, cy, tt);
 #pragma unroll
                 for (int n = 0; n < 5; n++) F[n] = fscale(double(fx[n]));
-                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc);
-                if (vh > a.ngridmax && owned(cx - 1, cy, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc);
+                const int sub = (cy & 1) + 2 * (tt & 1);
+                if (vl > a.ngridmax && owned(cx, cy, tt, vh))
+                    reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc, a.rdet, 4 + sub);
+                if (vh > a.ngridmax && owned(cx - 1, cy, tt, vl))
+                    reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc, a.rdet, sub);
             }
             if (wfy && !(myref || sref[s0][pc - PX])) {
                 const int vl = wv(cx, cy - 1, tt), vh = wv(cx, cy, tt);
 #pragma unroll
                 for (int n = 0; n < 5; n++) F[n] = fscale(double(fy[n]));
-                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc);
-                if (vh > a.ngridmax && owned(cx, cy - 1, tt, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc);
+                const int sub = (cx & 1) + 2 * (tt & 1);
+                if (vl > a.ngridmax && owned(cx, cy, tt, vh))
+                    reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc, a.rdet, 12 + sub);
+                if (vh > a.ngridmax && owned(cx, cy - 1, tt, vl))
+                    reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc, a.rdet, 8 + sub);
             }
             if (interior && tt >= tlo && tt <= thi + 1 && !(myref || ref_prev)) {
                 const int vl = wv(cx, cy, tt - 1), vh = wv(cx, cy, tt);
 #pragma unroll
                 for (int n = 0; n < 5; n++) F[n] = fscale(double(fz[n]));
-                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc);
-                if (vh > a.ngridmax && owned(cx, cy, tt - 1, vl)) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc);
+                const int sub = (cx & 1) + 2 * (cy & 1);
+                if (vl > a.ngridmax && owned(cx, cy, tt, vh))
+                    reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc, a.rdet, 20 + sub);
+                if (vh > a.ngridmax && owned(cx, cy, tt - 1, vl))
+                    reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc, a.rdet, 16 + sub);
             }
         }
         if constexpr (!SEP) __syncthreads();  // the fluxes overwrite the face states in sf
@@ -911,9 +905,27 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                     double *po = a.unew + (size_t)(v - 1) * 40 + ind;
                     double un[5];
 #pragma unroll
-                    for (int n = 0; n < 5; n++) {
+                    for (int n = 0; n < 5; n++)
                         un[n] = __builtin_fma(dtdx64, double(fz_prev[n]) - double(fz[n]), part_prev[n]);
-                        po[8 * n] = un[n];
+                    if constexpr (std::remove_reference_t<decltype(sm)>::PW) {
+                        // the two z-planes of an oct fill each 64-byte line of
+                        // unew: the lower plane's values wait in LDS (the
+                        // thread's own slot) and both halves are stored
+                        // together with the upper plane
+                        const int ic = cy * (2 * BX) + cx;
+                        if (tf & 1) {
+#pragma unroll
+                            for (int n = 0; n < 5; n++) {
+                                po[8 * n - 4] = sm.sun[n][ic];
+                                po[8 * n] = un[n];
+                            }
+                        } else {
+#pragma unroll
+                            for (int n = 0; n < 5; n++) sm.sun[n][ic] = un[n];
+                        }
+                    } else {
+#pragma unroll
+                        for (int n = 0; n < 5; n++) po[8 * n] = un[n];
                     }
                     if (CU && a.courant && !ref_prev) {
                         if constexpr (sizeof(T) == 4)
@@ -933,6 +945,17 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         for (int n = 0; n < 5; n++) qmz_prev[n] = qm[2][n];
         ref_prev = myref;
         rotate(qcol_p);
+    };
+    if constexpr (RH_MARCH_UNROLL2 && NS == 2) {
+        // two planes per trip (the trip count 2(zhi - zlo + 2) is even): the
+        // ring slots are constants and the registers carried from plane to
+        // plane are renamed instead of copied
+        for (int t = tlo; t <= thi + 2; t += 2) {
+            march(t, 0);
+            march(t + 1, 1);
+        }
+    } else {
+        for (int t = tlo; t <= thi + 2; ++t) march(t, t & 1);
     }
     if (a.courant) {
         // the brick's partials of courant_fine in a fixed reduction order
@@ -974,10 +997,11 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 ei += red[4 * i + 2];
                 d = red[4 * i + 3] < d ? red[4 * i + 3] : d;
             }
-            a.cpart[4 * bid + 0] = m;
-            a.cpart[4 * bid + 1] = e;
-            a.cpart[4 * bid + 2] = ei;
-            a.cpart[4 * bid + 3] = d;
+            const int ps = pslot >= 0 ? pslot : bid;
+            a.cpart[4 * ps + 0] = m;
+            a.cpart[4 * ps + 1] = e;
+            a.cpart[4 * ps + 2] = ei;
+            a.cpart[4 * ps + 3] = d;
         }
     }
 }
@@ -1002,13 +1026,33 @@ __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2),
     // XCD-aware brick assignment: consecutive bricks to the same XCD so that
     // their shared halo octs meet in that XCD's L2.
     int bid = blockIdx.x;
-    if (a.xcd_remap) {
+    int part = -1, pslot = bid;
+    if (a.xcd_remap && a.nsplit == 0) {
         const int nb = gridDim.x, x = bid & 7, l = bid >> 3;
         const int q = nb >> 3, r = nb & 7;
         bid = x * q + (x < r ? x : r) + l;
+        pslot = bid;
+    } else if (a.xcd_remap) {
+        // the last nsplit bricks of each XCD's range run as two work items,
+        // one per half of the brick's z layers, so that the final round of
+        // workgroups on the XCD is half as long
+        const int x = blockIdx.x & 7, l = blockIdx.x >> 3;
+        const int q = a.nbricks >> 3, r = a.nbricks & 7, m = a.nsplit;
+        const int start = x * q + (x < r ? x : r), count = q + (x < r ? 1 : 0);
+        if (l >= count + m) return;
+        if (l < count - m) {
+            bid = start + l;
+            pslot = bid;
+        } else {
+            const int j = l - (count - m);
+            bid = start + count - m + (j >> 1);
+            part = j & 1;
+            // Courant partial of the upper half: after the nbricks whole-brick slots
+            pslot = part == 0 ? bid : a.nbricks + x * m + (j >> 1);
+        }
     }
     if (bid >= a.nbricks) return;
-    brick_body<BX, BY, BZ, T, RS, NS, false>(a, bid, sm);
+    brick_body<BX, BY, BZ, T, RS, NS, false>(a, bid, sm, part, pslot);
 }
 
 // Work-list kernel: work item i = (descriptor, brick), where a descriptor
@@ -1099,8 +1143,8 @@ __global__ void __launch_bounds__(256) k_godunov1d(const God1Args a)
     if (a.reflux) {
         // the left neighbour cell lies in this oct (i2=1) or in slot 0; the right one in this oct or slot 2
         const int vl = i2 == 1 ? o : nb[0], vr = i2 == 0 ? o : nb[2];
-        if (!zl && vl > a.ngridmax) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, FLs, 2);
-        if (!zr && vr > a.ngridmax) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vr, 1.0, FRs, 2);
+        if (!zl && vl > a.ngridmax) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, FLs, 2, a.racc, a.rdet, 4);
+        if (!zr && vr > a.ngridmax) reflux_add(a.unew, a.ghost_parent, a.ngridmax, vr, 1.0, FRs, 2, a.racc, a.rdet, 0);
     }
 }
 
@@ -1239,28 +1283,44 @@ __global__ void k_zero(double *p, long n)
 // accumulator is left zeroed for the next step (it starts zeroed); ghosts
 // that received no flux leave their parent cell untouched, as the direct
 // reflux of the per-level calls does.
+// With nslot = kRefluxSlots (deterministic mode) each ghost has one slot per
+// face of its oct, added to the parent cell one after another in slot order:
+// the same sums on every run.
 __global__ void k_reflux_fold(double *unew, double *__restrict__ racc, const int32_t *__restrict__ ghost_parent,
-                              const int32_t *__restrict__ order, int nghost, int ttd)
+                              const int32_t *__restrict__ order, int nghost, int ttd, int nslot)
 {
     // in parent-cell order (see k_refresh_ghosts)
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= nghost) return;
     const int g = order[t];
-    double f[5];
-    bool any = false;
-#pragma unroll
-    for (int n = 0; n < 5; n++) {
-        f[n] = racc[5 * (size_t)g + n];
-        any = any || f[n] != 0.0;
-    }
-    if (!any) return;
-#pragma unroll
-    for (int n = 0; n < 5; n++) racc[5 * (size_t)g + n] = 0.0;
+    double *rg = racc + 5 * (size_t)g * nslot;
     const int igrid = ghost_parent[2 * g], icell = ghost_parent[2 * g + 1];
-    if (igrid <= 0) return;
-    double *p = unew + (size_t)(igrid - 1) * 5 * ttd + (icell - 1);
+    double *p = igrid > 0 ? unew + (size_t)(igrid - 1) * 5 * ttd + (icell - 1) : nullptr;
+    double acc[5];
+    bool touched = false;
+    for (int k = 0; k < nslot; k++) {
+        double f[5];
+        bool any = false;
 #pragma unroll
-    for (int n = 0; n < 5; n++) p[n * ttd] = p[n * ttd] + f[n];
+        for (int n = 0; n < 5; n++) {
+            f[n] = rg[5 * k + n];
+            any = any || f[n] != 0.0;
+        }
+        if (!any) continue;
+#pragma unroll
+        for (int n = 0; n < 5; n++) rg[5 * k + n] = 0.0;
+        if (!p) continue;  // parent in a boundary oct: the reference drops it with the cache oct
+        if (!touched) {
+#pragma unroll
+            for (int n = 0; n < 5; n++) acc[n] = p[n * ttd];
+            touched = true;
+        }
+#pragma unroll
+        for (int n = 0; n < 5; n++) acc[n] = acc[n] + f[n];
+    }
+    if (touched)
+#pragma unroll
+        for (int n = 0; n < 5; n++) p[n * ttd] = acc[n];
 }
 
 // Fixed-order reduction of the per-block partials by one 256-thread block:
@@ -1565,29 +1625,76 @@ static void launch_ev(K kernel, dim3 g, dim3 b, hipStream_t s, hipEvent_t e0, hi
         hipLaunchKernelGGL(kernel, g, b, 0, s, args...);
 }
 
-hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, hipStream_t s, hipEvent_t e0,
-                            hipEvent_t e1)
+// Half-brick split of the final round (GodArgs::nsplit).  With equal work
+// per brick, q bricks per XCD on `slots` concurrent workgroups per XCD run in
+// ceil(q / slots) rounds; when the last round holds rem = q mod slots <=
+// slots/2 bricks, running those rem bricks as 2 rem half-bricks ends the
+// launch about half a round earlier (e.g. 352 bricks per XCD on 64 slots:
+// 5.53 instead of 6 rounds, by a list-scheduling model).
+#ifndef RH_TAIL_SPLIT
+#define RH_TAIL_SPLIT 1
+#endif
+template <typename K> static int tail_split(K kernel, int nthreads, const GodArgs &a)
 {
+    if (!RH_TAIL_SPLIT || !a.xcd_remap || a.reflux) return 0;
+    static std::map<const void *, int> occ_cache;
+    static int cus_per_xcd = 0;
+    if (!cus_per_xcd) {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 0;
+        cus_per_xcd = ncu / 8 > 0 ? ncu / 8 : 1;
+    }
+    auto it = occ_cache.find((const void *)kernel);
+    int occ;
+    if (it == occ_cache.end()) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, nthreads, 0) != hipSuccess) occ = 0;
+        occ_cache[(const void *)kernel] = occ;
+    } else {
+        occ = it->second;
+    }
+    const int slots = occ * cus_per_xcd, q = a.nbricks >> 3;
+    if (slots <= 1 || q <= 0) return 0;
+    const int rem = q % slots;
+    return rem > 0 && 2 * rem <= slots ? rem : 0;
+}
+
+template <typename K>
+static void launch_brick(K kernel, int nt, const GodArgs &a0, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                         int *nparts)
+{
+    GodArgs a = a0;
+    a.nsplit = tail_split(kernel, nt, a);
+    const int q = a.nbricks >> 3, r = a.nbricks & 7;
+    const dim3 g(a.nsplit ? 8 * (q + (r ? 1 : 0) + a.nsplit) : a.nbricks), b(nt);
+    if (nparts) *nparts = a.nbricks + 8 * a.nsplit;
+    launch_ev(kernel, g, b, s, e0, e1, a);
+}
+
+hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, hipStream_t s, hipEvent_t e0,
+                            hipEvent_t e1, int *nparts)
+{
+    if (nparts) *nparts = a.nbricks > 0 ? a.nbricks : 0;
     if (a.nbricks <= 0) return hipSuccess;
 #define RH_LAUNCH_SHAPE(A, B, C)                                                                         \
     if (bx == A && by == B && bz == C) {                                                                 \
         constexpr int nt = (2 * A + 2) * (2 * B + 2);                                                     \
-        const dim3 g(a.nbricks), b(nt);                                                                  \
         if (a.slope == 3) {                                                                              \
             if (prec == 0)                                                                               \
-                launch_ev(k_godunov3d<A, B, C, double, 0, 3>, g, b, s, e0, e1, a);                       \
+                launch_brick(k_godunov3d<A, B, C, double, 0, 3>, nt, a, s, e0, e1, nparts);              \
             else                                                                                         \
-                launch_ev(k_godunov3d<A, B, C, float, 0, 3>, g, b, s, e0, e1, a);                        \
+                launch_brick(k_godunov3d<A, B, C, float, 0, 3>, nt, a, s, e0, e1, nparts);               \
         } else if (a.riemann == 3) {                                                                     \
             if (prec == 0)                                                                               \
-                launch_ev(k_godunov3d<A, B, C, double, 3, 2>, g, b, s, e0, e1, a);                       \
+                launch_brick(k_godunov3d<A, B, C, double, 3, 2>, nt, a, s, e0, e1, nparts);              \
             else                                                                                         \
-                launch_ev(k_godunov3d<A, B, C, float, 3, 2>, g, b, s, e0, e1, a);                        \
+                launch_brick(k_godunov3d<A, B, C, float, 3, 2>, nt, a, s, e0, e1, nparts);               \
         } else {                                                                                         \
             if (prec == 0)                                                                               \
-                launch_ev(k_godunov3d<A, B, C, double, 0, 2>, g, b, s, e0, e1, a);                       \
+                launch_brick(k_godunov3d<A, B, C, double, 0, 2>, nt, a, s, e0, e1, nparts);              \
             else                                                                                         \
-                launch_ev(k_godunov3d<A, B, C, float, 0, 2>, g, b, s, e0, e1, a);                        \
+                launch_brick(k_godunov3d<A, B, C, float, 0, 2>, nt, a, s, e0, e1, nparts);               \
         }                                                                                                \
         return hipGetLastError();                                                                        \
     }
@@ -1696,11 +1803,11 @@ hipError_t launch_zero(double *p, long n, hipStream_t s)
 }
 
 hipError_t launch_reflux_fold(double *unew, double *racc, const int32_t *ghost_parent, const int32_t *order,
-                              int nghost, int ttd, hipStream_t s)
+                              int nghost, int ttd, hipStream_t s, int nslot)
 {
     if (nghost <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_reflux_fold, dim3((nghost + 255) / 256), dim3(256), 0, s, unew, racc, ghost_parent, order,
-                       nghost, ttd);
+                       nghost, ttd, nslot);
     return hipGetLastError();
 }
 

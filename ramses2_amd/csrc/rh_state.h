@@ -37,6 +37,7 @@ struct LevelBricks {
     int32_t *org = nullptr;   // [nbricks][3] oct key of the brick's first oct
     int32_t *zr = nullptr;    // [nbricks] z layers holding octs of the level: zlo | zhi << 16
     int any_ref = 1;          // some cell of the level's octs is refined
+    int bnd = 1;              // some dimension is not periodic (boundary ghosts in the windows)
     int boxmin[3] = {0, 0, 0};
     int boxmax[3] = {0, 0, 0};
 };
@@ -74,7 +75,8 @@ struct AmrPlan {
         int first, count;          // items [first, first+count) of d_items
     };
     std::vector<Group> groups;
-    double *racc = nullptr;        // [max(nghost_total,1)][5] reflux sums per ghost oct
+    double *racc = nullptr;        // [max(nghost_total,1)][nslot][5] reflux sums per ghost oct
+    int nslot = 1;                 // 1, or kRefluxSlotsPerGhost in deterministic mode
     std::vector<int> levels;       // levels with octs, coarse to fine
 };
 
@@ -107,9 +109,16 @@ struct State {
     int brick_bx = 8, brick_by = 6, brick_bz = 8;
     // Godunov precision: 0 FP64, 1 FP32 reconstruction/fluxes + FP64 accumulation
     int precision = 0;
-    // Courant pass of the next step fused into the Godunov kernel's epilogue:
-    // -1 automatic (fused for the mixed precision kernel only), 0 off, 1 on
+    // Courant pass of the next step fused into the Godunov kernel (rh_level_run):
+    // -1 automatic (fused, in both precisions), 0 off, 1 on
     int fused_courant = -1;
+    // rh_level_run's captured pairs of steps, kept for later calls with the
+    // same arguments, arrays and tables (key: the bits of all of them); two
+    // entries, one per order of the two state arrays (an odd step count
+    // leaves them swapped)
+    std::vector<uint64_t> graph_key[2];
+    hipGraph_t level_graph[2] = {nullptr, nullptr};
+    hipGraphExec_t level_exec[2] = {nullptr, nullptr};
     // timer events
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // single-level device time stepping: dt and accumulated time on the device
@@ -123,6 +132,13 @@ struct State {
     int32_t *ghost_order = nullptr;   // [ncachemax] ghost indices, each level's range in parent-cell order
     int32_t *ghost_key = nullptr;     // [ncachemax][3]
     int32_t *d_counter = nullptr;     // device scratch counters [4]
+    int32_t *flag1 = nullptr;         // [ngridmax][8] refinement flags (allocated at first use)
+    // deterministic refluxes (rh_set_deterministic): each coarse-fine face
+    // writes its own slot, folded in a fixed order; racc_det holds the slots
+    // of the per-level drop-in calls
+    int deterministic = 0;
+    double *racc_det = nullptr;
+    long racc_det_cap = 0;            // ghosts it has room for
     int interpol_var = 0, interpol_type = 1;
     // reduction scratch
     double *red_part = nullptr;
@@ -151,6 +167,8 @@ int fill_args1(int head_idx, int num_octs, int ilevel, double dx, double gamma, 
                God1Args &a);
 // Grow the current context's per-brick partial-sum buffer to n entries.
 int ensure_partials(int n);
+// Destroy the context's cached rh_level_run graph (its arrays or tables change).
+void drop_level_graph(State &s);
 
 // The current context (the drop-in entry points act on it).
 State &st();

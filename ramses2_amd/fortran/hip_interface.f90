@@ -90,6 +90,38 @@ module hip_interface
       real(c_double), intent(in) :: constant_gravity(3)
     end function rh_godunov
 
+    ! refinement flags (mtl_upload_flag1 / mtl_reset_flag1 / mtl_count_flag1 /
+    ! mtl_hydro_flag, metal/metal_interface.f90:59-63,200-241)
+    integer(c_int) function rh_upload_flag1(flag1, ngridmax) bind(C, name="rh_upload_flag1")
+      import c_int
+      integer(c_int), value :: ngridmax
+      integer(c_int), intent(in) :: flag1(*)
+    end function rh_upload_flag1
+
+    integer(c_int) function rh_transfer_flag1_host(flag1, ngridmax) bind(C, name="rh_transfer_flag1_host")
+      import c_int
+      integer(c_int), value :: ngridmax
+      integer(c_int), intent(out) :: flag1(*)
+    end function rh_transfer_flag1_host
+
+    integer(c_int) function rh_reset_flag1(head_idx, num_octs) bind(C, name="rh_reset_flag1")
+      import c_int
+      integer(c_int), value :: head_idx, num_octs
+    end function rh_reset_flag1
+
+    integer(c_int) function rh_count_flag1(head_idx, num_octs, count) bind(C, name="rh_count_flag1")
+      import c_int
+      integer(c_int), value :: head_idx, num_octs
+      integer(c_int), intent(out) :: count
+    end function rh_count_flag1
+
+    integer(c_int) function rh_hydro_flag(head_idx, num_octs, gamma, smallr, smallc2, err_grad_d, err_grad_p, &
+         floor_d, floor_p) bind(C, name="rh_hydro_flag")
+      import c_int, c_double
+      integer(c_int), value :: head_idx, num_octs
+      real(c_double), value :: gamma, smallr, smallc2, err_grad_d, err_grad_p, floor_d, floor_p
+    end function rh_hydro_flag
+
     integer(c_int) function rh_device_sync() bind(C, name="rh_device_sync")
       import c_int
     end function rh_device_sync

@@ -95,6 +95,31 @@ class HipHydro:
         call("rh_build_nbor", head, num, self.hash_size, int(self.ckey_max[l]), int(self.key_off[l]),
              ptr(bmin, _lib._ip), ptr(bmax, _lib._ip), ptr(self.periodic, _lib._ip))
 
+    def hydro_flag(self, head, num, p: HydroParams, err_grad_d, err_grad_p, floor_d=1e-10, floor_p=1e-10,
+                   reset=True):
+        """Refinement flags of octs head..head+num-1 from the device state
+        (rh_hydro_flag, flag1 cleared first when reset): returns flag1
+        (num, 8) int32 and the number of flagged cells."""
+        if reset:
+            call("rh_reset_flag1", head, num)
+        call("rh_hydro_flag", head, num, p.gamma, p.smallr, p.smallc ** 2, float(err_grad_d), float(err_grad_p),
+             float(floor_d), float(floor_p))
+        n = ctypes.c_int32()
+        call("rh_count_flag1", head, num, ctypes.byref(n))
+        f = np.zeros((self.ngrid, 8), dtype=np.int32)
+        call("rh_transfer_flag1_host", ptr(f, _lib._ip), self.ngrid)
+        return f[head - 1:head - 1 + num], n.value
+
+    def level_table(self, ilevel, which):
+        """One of the level's device index tables (rh_get_level_table) as a
+        flat int32 array."""
+        n = ctypes.c_int64()
+        L = _lib.load()
+        L.rh_get_level_table(ilevel, which, None, 0, ctypes.byref(n))
+        out = np.zeros(max(n.value, 1), dtype=np.int32)
+        call("rh_get_level_table", ilevel, which, ptr(out, _lib._ip), n.value, ctypes.byref(n))
+        return out[:n.value]
+
     # ---- level kernels ----
     def set_unew(self, head, num):
         call("rh_set_unew", head, num)
@@ -238,17 +263,21 @@ class AmrHierarchy:
     All levels share dt = min over levels (amr_step.f90:588)."""
 
     def __init__(self, amesh, uold, params: HydroParams, ncachemax=None, interpol_var=0, interpol_type=1,
-                 precision=0, bricks=None):
+                 precision=0, bricks=None, periodic=(1, 1, 1), bound_type=(2, 2, 2, 2, 2, 2), deterministic=False):
         """bricks: {level: (bx, by, bz)} brick shape per level (default: the
-        library's current shape for every level)."""
+        library's current shape for every level).  periodic: per dimension;
+        bound_type: (x low, x high, y low, ...) of the non-periodic faces, 1
+        reflexive, 2 zero gradient.  deterministic: coarse-fine refluxes
+        folded in a fixed order (rh_set_deterministic) instead of atomics."""
         self.am = amesh
         self.p = params
         self.precision = precision
         nc = ncachemax if ncachemax is not None else max(4 * amesh.ngrid, 10000)
         self.hh = HipHydro(3, amesh.ngrid, nc)
         ckey_max, key_off, box_min, box_max = amesh_level_params(amesh)
-        self.hh.upload_level_params(ckey_max, key_off, box_min, box_max, [1, 1, 1])
-        self.hh.set_boundaries([2] * 6)
+        self.hh.upload_level_params(ckey_max, key_off, box_min, box_max, list(periodic))
+        self.hh.set_boundaries(list(bound_type))
+        call("rh_set_deterministic", int(bool(deterministic)))
         call("rh_set_interpolation", int(interpol_var), int(interpol_type))
         self.interpol_var = interpol_var
         self.hh.set_grid_device(uold, uold, amesh.grid_records())
@@ -309,5 +338,4 @@ class AmrHierarchy:
 
 
 def amesh_level_params(amesh):
-    from .mesh import level_params
-    return level_params(amesh.levelmax, 3)
+    return amesh.level_params()

@@ -169,10 +169,14 @@ class AmrMesh:
     level, refined(ind) set where the child oct exists (an oct of level l+1
     with key K is the child of the level-l cell with coordinates K)."""
 
-    def __init__(self, ndim, level_keys, boxlen=1.0):
+    def __init__(self, ndim, level_keys, boxlen=1.0, box=None):
+        """box = (lev, ckey_min[3], ckey_max[3]): a computational box smaller
+        than the full levels (the reference's box_ckey_min/max, given at level
+        lev and scaled by 2 per level); None for the full levels."""
         assert ndim == 3, "AmrMesh builds 3D meshes"
         self.ndim = ndim
         self.boxlen = boxlen
+        self.box = box
         self.ttd = 8
         self.levels = sorted(level_keys)
         self.levelmin, self.levelmax = self.levels[0], self.levels[-1]
@@ -230,6 +234,19 @@ class AmrMesh:
 
     def dx(self, l):
         return self.boxlen / 2 ** l
+
+    def level_params(self):
+        """ckey_max, key_off, box_ckey_min/max per level 1..levelmax+1, the
+        box scaled from its level by powers of two."""
+        ckey_max, key_off, box_min, box_max = level_params(self.levelmax, self.ndim)
+        if self.box is not None:
+            bl, bmin, bmax = self.box
+            for l in range(1, len(ckey_max) + 1):
+                sh = l - bl
+                for d in range(self.ndim):
+                    box_min[l - 1, d] = bmin[d] << sh if sh >= 0 else bmin[d] >> -sh
+                    box_max[l - 1, d] = bmax[d] << sh if sh >= 0 else max(1, bmax[d] >> -sh)
+        return ckey_max, key_off, box_min, box_max
 
     def grid_records(self):
         g = np.zeros(self.ngrid, dtype=OCT3)

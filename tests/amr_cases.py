@@ -33,6 +33,20 @@ def three_level_mesh(lo=4):
     return AmrMesh(3, {lo: k4, lo + 1: k5, lo + 2: k6})
 
 
+def low_face_mesh(shift=0, lo=4):
+    """A box of 4 x 8 x 8 octs at level lo, non-periodic in x, whose x keys
+    start at `shift` (the reference's non-periodic boxes start at a positive
+    key, e.g. box_xmin=1); a refined block of level lo+1 touching the low x
+    face: x keys [2 shift, 2 shift + 4), y and z keys [4, 12).  Its ghost octs
+    next to that face take neighbouring parent cells from the coarse level's
+    boundary octs."""
+    k4 = np.stack(np.meshgrid(np.arange(shift, shift + 4), np.arange(8), np.arange(8), indexing="ij"),
+                  -1).reshape(-1, 3)
+    jj = np.arange(4, 12)
+    k5 = np.stack(np.meshgrid(np.arange(2 * shift, 2 * shift + 4), jj, jj, indexing="ij"), -1).reshape(-1, 3)
+    return AmrMesh(3, {lo: k4, lo + 1: k5}, box=(lo, [shift, 0, 0], [shift + 4, 8, 8]))
+
+
 def smooth_state(am, seed, gamma=1.4, amp=0.3, noise=0.02, blast=False, shift_x=0.0):
     rng = np.random.default_rng(seed)
     u = np.zeros((am.ngrid, 5, 8))
@@ -54,10 +68,16 @@ def smooth_state(am, seed, gamma=1.4, amp=0.3, noise=0.02, blast=False, shift_x=
     return u
 
 
-def oracle_mesh(am, u, interpol_var=0, interpol_type=1):
+def oracle_mesh(am, u, interpol_var=0, interpol_type=1, periodic=(1, 1, 1), bound_type=((2, 2), (2, 2), (2, 2))):
     head, noct = am.heads()
     return O.OracleMesh(3, am.ckey, am.lev, am.refined, u, head, noct, levelmin=am.levelmin,
-                        boxlen=am.boxlen, interpol_var=interpol_var, interpol_type=interpol_type)
+                        boxlen=am.boxlen, interpol_var=interpol_var, interpol_type=interpol_type,
+                        periodic=periodic, bound_type=bound_type, box=am.box)
+
+
+def key_index(am):
+    """{(level, key tuple): 0-based oct index} of a mesh."""
+    return {(int(am.lev[i]), tuple(int(x) for x in am.ckey[i])): i for i in range(am.ngrid)}
 
 
 def restrict_all(am, om):

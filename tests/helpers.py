@@ -36,6 +36,20 @@ def rel_err_per_var(a, b):
     return np.array(out)
 
 
+def rel_err_per_cell(a, b, floor=1e-6):
+    """max over cells of |a-b| / max(|b|, floor * max_cells |b|) for each
+    conserved variable: the relative error of every cell, with a floor that
+    keeps components crossing zero (momenta) from dividing by ~0."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    out = []
+    for v in range(a.shape[1]):
+        s = np.max(np.abs(b[:, v]))
+        den = np.maximum(np.abs(b[:, v]), floor * s if s > 0 else 1.0)
+        out.append(np.max(np.abs(a[:, v] - b[:, v]) / den))
+    return np.array(out)
+
+
 def l1_rel_per_var(a, b):
     """sum |a-b| / sum |b| per conserved variable."""
     out = []

@@ -2,19 +2,22 @@
 
 Bar: the FP64 kernel within 1e-12 of the oracle per conserved variable, measured
 as max_cells|gpu-oracle| / max_cells|oracle| for each variable, on one step
-with the same dt (FMA contraction is the only allowed difference).  The mixed
-kernel (FP32 reconstruction/fluxes, FP64 accumulation) within 2e-6.
+with the same dt (FMA contraction is the only allowed difference), and within
+TOLCELL of it cell by cell: |gpu-oracle| / max(|oracle|, 1e-6 max_cells|oracle|)
+for every cell and variable.  The mixed kernel (FP32 reconstruction/fluxes,
+FP64 accumulation) within 2e-6.
 """
 import numpy as np
 import pytest
 
 from ramses2_amd import HydroParams, UniformLevel, UniformMesh, ic
 from ramses2_amd.autotune import DEFAULT_SHAPE
-from tests.helpers import oracle_params, oracle_uniform, rel_err_per_var
+from tests.helpers import oracle_params, oracle_uniform, rel_err_per_cell, rel_err_per_var
 
 pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-12
+TOLCELL = 1e-11
 TOLMIX = 2e-6
 
 
@@ -48,6 +51,9 @@ def test_godunov3d_random_fp64(slope, riemann):
     g, o, _, _, _ = _one_step_3d(mesh_level, cells, p)
     err = rel_err_per_var(g, o)
     assert np.all(err < TOL64), err
+    cell = rel_err_per_cell(g, o)
+    print("per-cell relative error", cell)
+    assert np.all(cell < TOLCELL), cell
 
 
 def test_godunov3d_floors_fp64():
@@ -79,6 +85,7 @@ def test_brick_shapes_agree(brick, variant):
         from ramses2_amd.hydro import HipHydro
         HipHydro.set_brick_shape(*DEFAULT_SHAPE)
     assert np.all(rel_err_per_var(g, o) < TOL64)
+    assert np.all(rel_err_per_cell(g, o) < TOLCELL), rel_err_per_cell(g, o)
 
 
 @pytest.mark.parametrize("precision", [0, 1])
@@ -227,3 +234,33 @@ def test_conservation_periodic():
         s0, s1 = u0[:, v].sum(), u1[:, v].sum()
         scale = np.abs(u0[:, v]).sum()
         assert abs(s1 - s0) <= 1e-12 * scale, (v, s0, s1)
+
+
+@pytest.mark.parametrize("slope", [1, 2])
+def test_constant_gravity_fp64(slope):
+    """A constant gravity vector: ctoprim's g dt/2 velocity kick and cmpdt's
+    gravity term (hydro/umuscl.f90:1739-1901, hydro/godunov_utils.f90:228-337),
+    through the drop-in calls, against the oracle (itself pinned to the
+    reference's godunov_fine with gravity)."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(21 + slope)
+    mesh = UniformMesh(3, 4)
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=slope, riemann="hllc")
+    g = (0.7, -0.4, 0.25)
+    lev = UniformLevel(mesh, ic.random_state((16, 16, 16), rng), p)
+    u0 = lev.state()
+    om = O.OracleMesh(3, mesh.ckey, np.full(mesh.noct, 4), np.zeros((mesh.noct, 8)), u0, [0, 0, 0, 1],
+                      [0, 0, 0, mesh.noct], levelmin=4, gravity=g)
+    po = oracle_params(p)
+    c_o = om.courant_fine(4, po)
+    c_g = lev.hh.cmpdt(1, mesh.noct, mesh.dx, p, gravity=g)
+    assert abs(c_g["dt"] - c_o[4]) <= 1e-14 * c_o[4]
+    dt = c_o[4]
+    lev.hh.set_unew(1, mesh.noct)
+    lev.hh.godunov(1, mesh.noct, 4, 4, 4, dt, mesh.dx, p, gravity=g)
+    lev.hh.set_uold(1, mesh.noct)
+    om.set_unew(4)
+    om.godunov_fine(4, dt, po)
+    err = rel_err_per_var(lev.state(), om.unew)
+    assert np.all(err < TOL64), err
+    assert np.all(rel_err_per_cell(lev.state(), om.unew) < TOLCELL)

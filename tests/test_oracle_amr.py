@@ -65,3 +65,39 @@ def test_amr_shift_invariance():
     for l in res[0]:
         a, b = res[0][l], res[1][l]
         assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(a)), l
+
+
+@pytest.mark.parametrize("btype", [(2, 2), (1, 1)])
+def test_nonperiodic_low_face_is_shift_invariant(btype):
+    """A refined block touching the low face of a box non-periodic in x.  The
+    reference's non-periodic boxes start at a positive key (box_xmin=1,
+    amr/init_amr.f90:255-283), where its integer division of parent keys
+    (amr/nbors_utils.f90:180,200) never meets a negative key; a box that
+    starts at key 0 needs the key -1 beside it to map to the coarse boundary
+    oct -1, cell bit 1 (floor division).  Two steps of the box from key 0
+    must equal, cell for cell, two steps of the same box shifted by 2 coarse
+    octs, whose keys are all positive; a truncating division would take the
+    ghosts' low neighbours from the wrong parent cells (cell bit -1)."""
+    from tests.amr_cases import key_index, low_face_mesh
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
+    bt = (btype, (2, 2), (2, 2))
+    a0, a2 = low_face_mesh(0), low_face_mesh(2)
+    u0 = smooth_state(a0, 21, blast=True)
+    k0, k2 = key_index(a0), key_index(a2)
+    # the same state on the shifted mesh, oct for oct by (level, key)
+    u2 = np.zeros_like(u0)
+    for (l, k), i in k0.items():
+        s = 2 if l == a0.levelmin else 4
+        u2[k2[(l, (k[0] + s, k[1], k[2]))]] = u0[i]
+    om0 = oracle_mesh(a0, u0, 0, 1, periodic=(0, 1, 1), bound_type=bt)
+    om2 = oracle_mesh(a2, u2, 0, 1, periodic=(0, 1, 1), bound_type=bt)
+    restrict_all(a0, om0)
+    restrict_all(a2, om2)
+    for _ in range(2):
+        d0 = oracle_step(a0, om0, p)
+        d2 = oracle_step(a2, om2, p)
+        assert d0 == d2
+    for (l, k), i in k0.items():
+        s = 2 if l == a0.levelmin else 4
+        j = k2[(l, (k[0] + s, k[1], k[2]))]
+        assert np.array_equal(om0.uold[i], om2.uold[j]), (l, k)

@@ -247,3 +247,25 @@ def test_sub_box_equals_full_level():
         om.godunov_fine(m.level, 1e-3, oracle_params(p))
         out.append(m.to_cells(om.unew))
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_uniform_omp_baseline_equals_godunov_fine(threads):
+    """The CPU baseline bench.py times (ora_godunov_uniform_omp: the same
+    per-oct arithmetic, OpenMP over octs, neighbours from an oct map instead
+    of the hash) gives godunov_fine's result bit for bit on a periodic level,
+    with one thread and with several."""
+    from ramses2_amd import UniformMesh, ic
+    from tests.helpers import oracle_params, oracle_uniform
+    mesh = UniformMesh(3, 4)
+    p = HydroParams(gamma=5.0 / 3.0, courant_factor=0.5, slope_type=2, riemann="hllc")
+    po = oracle_params(p)
+    u0 = mesh.from_cells(ic.turbulence_3d(16, seed=3))
+    a = oracle_uniform(mesh, u0.copy())
+    b = oracle_uniform(mesh, u0.copy())
+    dt = a.courant_fine(4, po)[4]
+    a.set_unew(4)
+    a.godunov_fine(4, dt, po)
+    b.set_unew(4)
+    b.godunov_uniform_omp(4, dt, po, (mesh.octmap + 1).astype(np.int32), threads)
+    assert np.array_equal(a.unew, b.unew)

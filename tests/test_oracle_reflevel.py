@@ -215,3 +215,38 @@ def test_boundary_ghosts_3d_faces_match_reference(dim, btype):
                 k.insert(dim, side)
                 keys.append(tuple(k))
     compare_probes(om, keys, 3)
+
+
+@pytest.mark.parametrize("mesh", ["two", "three", "uniform"])
+@pytest.mark.parametrize("crit", [(0.1, 0.1), (0.05, -1.0), (-1.0, 0.3)])
+def test_hydro_flag_matches_reference(mesh, crit):
+    """The refinement criterion (hydro_flag with hydro_refine's density and
+    pressure gradient tests, hydro/hydro_flag.f90:7-159) on every level of a
+    periodic mesh after two steps of a blast: the oracle's flag1 equals the
+    reference's cell for cell (neighbours of a level's cells taken from the
+    level or, where missing, from the coarser level's cell)."""
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
+    po = oracle_params(p)
+    if mesh == "uniform":
+        m = UniformMesh(3, 4)
+        om = oracle_uniform(m, m.from_cells(ic.sedov_3d(16)))
+        levels, dx = [4], (lambda l: m.dx)
+    else:
+        am = two_level_mesh() if mesh == "two" else three_level_mesh()
+        om = oracle_mesh(am, smooth_state(am, 5, blast=True), 0, 1)
+        levels, dx = am.levels, am.dx
+    ref = O.RefLevel(om, po)
+    if mesh != "uniform":
+        restrict_both(am, om, ref)
+    for _ in range(2):
+        step_both(om, ref, levels, po, p.smallc, dx)
+    assert_same(om, ref)
+    nflag = 0
+    for l in levels:
+        fr = ref.hydro_flag(l, crit[0], crit[1])
+        fo, n = om.hydro_flag(l, p.gamma, p.smallr, crit[0], crit[1])
+        assert np.array_equal(fo, fr), (l, np.argwhere(fo != fr)[:5])
+        assert n == int(fr.sum())
+        nflag += n
+    # the criterion flags some cells and leaves others
+    assert 0 < nflag < sum(om.noct) * 8
