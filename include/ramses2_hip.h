@@ -116,6 +116,20 @@ int rh_cmpdt(int head_idx, int num_octs, double dx, double gamma, double smallr,
              double smallc2, double courant_factor, const double g[3],
              double *mass, double *ekin, double *eint, double *emag, double *dt);
 
+/* No counterpart in the reference: rh_cmpdt for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts that
+ * forbid hipMalloc).  Same arguments and results as rh_cmpdt; its per-block
+ * partial sums and result live in a static device array of the library
+ * (4096 partials of 4 doubles + 8 doubles, 128 KB per GPU, held until the
+ * process ends) instead of library-allocated memory, and it calls no
+ * hipMalloc, hipFree or allocating routine.  The reduction covers any oct
+ * range (at most 4096 blocks, each striding over the cells).  Calls that
+ * overlap on one GPU share that array and race; nothing in it needs clearing
+ * (each call overwrites what it reads). */
+int rh_cmpdt_noalloc(int head_idx, int num_octs, double dx, double gamma, double smallr, double smallc2,
+                     double courant_factor, const double g[3], double *mass, double *ekin, double *eint,
+                     double *emag, double *dt);
+
 /* Replaces mtl_godunov (metal/metal_interface.f90:537-558), i.e. godunov_fine /
  * godfine1 / unsplit (hydro/godunov_fine.f90:43-79,244-1342, hydro/umuscl.f90:20-292):
  * unew += conservative update of every oct of the level, plus the refluxing of

@@ -32,6 +32,7 @@ SIGNATURES = {
     "rh_set_unew": [_c, _c],
     "rh_set_uold": [_c, _c],
     "rh_cmpdt": [_c, _c, _d, _d, _d, _d, _d, _dp, _dp, _dp, _dp, _dp, _dp],
+    "rh_cmpdt_noalloc": [_c, _c, _d, _d, _d, _d, _d, _dp, _dp, _dp, _dp, _dp, _dp],
     "rh_godunov": [_c, _c, _c, _c, _c, _c, _d, _d, _d, _d, _d, _d, _c, _c, _dp],
     "rh_device_sync": [],
     "rh_upload": [_c, _c, _c, _d, _d, _d],

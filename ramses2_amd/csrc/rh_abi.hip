@@ -375,6 +375,34 @@ extern "C" {
 
 const char *rh_last_error(void) { return st().last_error.c_str(); }
 
+// Scratch of rh_cmpdt_noalloc: Courant partials (kNoAllocParts x 4 doubles)
+// and the 5 results, one copy per GPU.  Host code uses the per-device
+// address from hipGetSymbolAddress (g_scratch_dev), never the symbol itself.
+static constexpr int kNoAllocParts = 4096;
+static __device__ __align__(256) double scratch[kNoAllocParts * 4 + 8];
+static double *g_scratch_dev[64];
+
+// The device address of `scratch` on the current device, fetched once per
+// device (with that device current); null and an error message on failure.
+static double *scratch_ptr()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        set_error("rh_cmpdt_noalloc: no current HIP device (or device id >= 64)");
+        return nullptr;
+    }
+    if (!g_scratch_dev[dev]) {
+        double *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("rh_cmpdt_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    return g_scratch_dev[dev];
+}
+
 int rh_init(void)
 {
     State &s = st();
@@ -395,6 +423,8 @@ int rh_init(void)
     RH_HIP_CHECK(dmalloc((void **)&s.t_dev, 2 * sizeof(double)));
     RH_HIP_CHECK(hipEventCreate(&s.ev0));
     RH_HIP_CHECK(hipEventCreate(&s.ev1));
+    // the no-allocation Courant entry point's scratch address on this device
+    if (!scratch_ptr()) return RH_ERR_HIP;
     s.inited = true;
     s.last_error.clear();
     return 0;
@@ -896,6 +926,34 @@ int rh_cmpdt(int head_idx, int num_octs, double dx, double gamma, double smallr,
     RH_HIP_CHECK(launch_courant(s.uold, s.refined, head_idx, num_octs, s.ndim, dx, gamma, smallr, smallc,
                                 courant_factor, g ? g : gz, s.red_part, s.red_cap, s.red_out, nullptr, s.stream));
     RH_HIP_CHECK(hipMemcpyAsync(s.h_red, s.red_out, 5 * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+    if (mass) *mass = s.h_red[0];
+    if (ekin) *ekin = s.h_red[1];
+    if (eint) *eint = s.h_red[2];
+    if (emag) *emag = s.h_red[3];
+    if (dt) *dt = s.h_red[4];
+    return 0;
+}
+
+int rh_cmpdt_noalloc(int head_idx, int num_octs, double dx, double gamma, double smallr, double smallc2,
+                     double courant_factor, const double g[3], double *mass, double *ekin, double *eint,
+                     double *emag, double *dt)
+{
+    if (int r = ensure_mesh()) return r;
+    State &s = st();
+    if (head_idx < 1 || num_octs < 0 || (int64_t)head_idx - 1 + num_octs > s.ngridtot)
+        return fail(RH_ERR_ARG, "rh_cmpdt_noalloc: oct range out of bounds");
+    if (!(dx > 0) || !(smallc2 > 0) || !(gamma > 1))
+        return fail(RH_ERR_ARG, "rh_cmpdt_noalloc: bad physical parameters");
+    double *sp = scratch_ptr();
+    if (!sp) return RH_ERR_HIP;
+    double *partials = sp, *out = sp + 4 * kNoAllocParts;
+    const double gz[3] = {0, 0, 0};
+    RH_HIP_CHECK(launch_courant(s.uold, s.refined, head_idx, num_octs, s.ndim, dx, gamma, smallr,
+                                __builtin_sqrt(smallc2), courant_factor, g ? g : gz, partials, kNoAllocParts, out,
+                                nullptr, s.stream));
+    // results through the pinned host buffer allocated at rh_init
+    RH_HIP_CHECK(hipMemcpyAsync(s.h_red, out, 5 * sizeof(double), hipMemcpyDeviceToHost, s.stream));
     RH_HIP_CHECK(hipStreamSynchronize(s.stream));
     if (mass) *mass = s.h_red[0];
     if (ekin) *ekin = s.h_red[1];
