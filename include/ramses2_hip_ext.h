@@ -26,6 +26,13 @@ int rh_set_precision(int precision);
  * as it stores it, the FP32 kernel in an epilogue that re-reads its cells). */
 int rh_set_fused_courant(int mode);
 
+/* Artificial diffusion coefficient (hydro_params_t%difmag, amr/amr_commons.f90:
+ * 690-702; the Metal ABI has no such argument): difmag > 0 adds the diffusive
+ * fluxes of cmpdivu/cmpdiff (hydro/umuscl.f90:2392-2540) to every 3D level
+ * step of rh_godunov and rh_level_run (with a separate Courant pass); the
+ * batched AMR and serving steps refuse it.  Default 0. */
+int rh_set_difmag(double difmag);
+
 /* Deterministic coarse-fine refluxes: 1 = each face between a level cell and
  * an interpolated ghost cell stores its flux in a slot of its own (24 per
  * ghost oct), and a fold adds the slots to the parent cell in a fixed order

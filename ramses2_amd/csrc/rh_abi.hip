@@ -1006,6 +1006,7 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         RH_HIP_CHECK(launch_refresh_ghosts(ga, s.stream));
     }
     const int reflux = ilevel > levelmin ? 1 : 0;
+    if (s.difmag > 0.0 && s.ndim != 3) return fail(RH_ERR_ARG, "difmag > 0 is supported in 3D only");
     // deterministic refluxes: per-face slots of the level's ghosts, folded into
     // the parent cells in slot order after the launch
     double *racc = nullptr;
@@ -1067,6 +1068,8 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         a.racc = racc;
         a.rdet = racc != nullptr;
         RH_HIP_CHECK(launch_godunov3d(a, lb.bx, lb.by, lb.bz, s.precision, s.stream));
+        RH_HIP_CHECK(enqueue_diffusion(s, ilevel, head_idx, num_subgrids, dx, smallr, gg, nullptr, dt, reflux, racc,
+                                       racc != nullptr));
     } else {
         const LevelNbor &ln = lnb;
         God1Args a;

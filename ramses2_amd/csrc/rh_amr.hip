@@ -106,6 +106,8 @@ int rh_amr_prepare(int levelmin, int levelmax, const int32_t *head, const int32_
     State &s = st();
     if (!s.inited || !s.uold) return efail(RH_ERR_STATE, "library not initialised / mesh not allocated");
     if (s.ndim != 3) return efail(RH_ERR_ARG, "batched AMR stepping is 3D");
+    if (s.difmag > 0.0)
+        return efail(RH_ERR_ARG, "difmag > 0 runs through rh_godunov and rh_level_run, not the batched AMR step");
     if (levelmin < 1 || levelmax < levelmin || levelmax > s.nlevelmax || !head || !noct)
         return efail(RH_ERR_ARG, "rh_amr_prepare: bad level range");
     if (riemann < 1 || riemann > 3 || slope < 0 || slope > 3 || !(boxlen > 0))

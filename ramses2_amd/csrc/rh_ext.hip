@@ -162,6 +162,45 @@ int ensure_partials(int n)
     return 0;
 }
 
+hipError_t enqueue_diffusion(State &s, int ilevel, int head_idx, int num_octs, double dx, double smallr,
+                             const double g[3], const double *dt_dev, double dt, int reflux, double *racc, int rdet)
+{
+    if (!(s.difmag > 0.0) || num_octs <= 0) return hipSuccess;
+    auto it = s.nbors.find(ilevel);
+    if (it == s.nbors.end()) return hipErrorInvalidValue;
+    const LevelNbor &ln = it->second;
+    DiffArgs d;
+    d.uold = s.uold;
+    d.unew = s.unew;
+    d.refined = s.refined;
+    d.nbor = ln.nbor;
+    d.ckey = s.ckey;
+    d.head = head_idx;
+    d.noct = num_octs;
+    d.ndim = s.ndim;
+    d.refl = 0;
+    for (int k = 0; k < 3; k++) {
+        d.boxmin[k] = ln.boxmin[k];
+        d.boxmax[k] = ln.boxmax[k];
+        for (int sd = 0; sd < 2; sd++) d.refl |= (s.btype[k][sd] == 1) << (2 * k + sd);
+        d.g[k] = g ? g[k] : 0.0;
+    }
+    d.dt_dev = dt_dev;
+    d.dt = dt;
+    d.dx = dx;
+    d.difmag = s.difmag;
+    d.smallr = smallr;
+    d.reflux = reflux;
+    d.ngridmax = s.ngridmax;
+    d.ghost_parent = s.ghost_parent;
+    d.racc = racc;
+    d.rdet = rdet;
+    return launch_diffusion3d(d, s.stream);
+}
+
+// One step.  3D with a Courant dt and fusion on: the Godunov kernel also
+// reduces the Courant condition of the state it writes (dt of the next step),
+// followed by a one-block final reduction; otherwise courant then godunov.
 }  // namespace rh
 
 namespace {
@@ -174,12 +213,11 @@ namespace {
 bool courant_fused()
 {
     const State &s = st();
+    // the diffusion pass changes the state after the Godunov kernel reduced it
+    if (s.difmag > 0.0) return false;
     return s.fused_courant < 0 ? true : s.fused_courant != 0;
 }
 
-// One step.  3D with a Courant dt and fusion on: the Godunov kernel also
-// reduces the Courant condition of the state it writes (dt of the next step),
-// followed by a one-block final reduction; otherwise courant then godunov.
 hipError_t enqueue_step(int head_idx, int num_octs, double dx, double gamma, double smallr, double smallc2,
                         double courant_factor, const double g[3], double dt_fixed, GodArgs *a3, God1Args *a1,
                         const LevelBricks *lb, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
@@ -210,6 +248,8 @@ hipError_t enqueue_step(int head_idx, int num_octs, double dx, double gamma, dou
         a3->unew = s.unew;
         a3->ubase = s.uold;
         e = launch_godunov3d(*a3, lb->bx, lb->by, lb->bz, s.precision, s.stream, e0, e1);
+        if (e == hipSuccess)
+            e = enqueue_diffusion(s, lb->ilevel, head_idx, num_octs, dx, smallr, g, a3->dt_dev, a3->dt, 0, nullptr, 0);
     } else {
         a1->uold = s.uold;
         a1->unew = s.unew;
@@ -250,6 +290,14 @@ int rh_set_brick_shape(int bx, int by, int bz)
     st().brick_bx = bx;
     st().brick_by = by;
     st().brick_bz = bz;
+    return 0;
+}
+
+int rh_set_difmag(double difmag)
+{
+    if (!(difmag >= 0.0)) return efail(RH_ERR_ARG, "rh_set_difmag: difmag must be >= 0");
+    st().difmag = difmag;
+    drop_level_graph(st());
     return 0;
 }
 
@@ -297,6 +345,7 @@ int rh_level_run(int head_idx, int num_octs, int ilevel, double dx, double gamma
     if (nsteps < 0) return efail(RH_ERR_ARG, "nsteps must be >= 0");
     if (!level_is_whole_mesh(head_idx, num_octs))
         return efail(RH_ERR_STATE, "rh_level_run needs a single-level mesh (use the drop-in calls for AMR)");
+    if (s.difmag > 0.0 && s.ndim != 3) return efail(RH_ERR_ARG, "difmag > 0 is supported in 3D only");
     GodArgs a3;
     God1Args a1;
     const LevelBricks *lb = nullptr;

@@ -167,6 +167,26 @@ hipError_t launch_refresh_ghosts(const GhostArgs &a, hipStream_t s);
 hipError_t launch_upload(double *uold, const int32_t *father, int head, int noct, int ndim, int internal_energy,
                          double smallr, hipStream_t s);
 
+// Artificial diffusion of a 3D level (difmag > 0), after its Godunov launch:
+// unew += the cmpdiff fluxes of uold (hydro/umuscl.f90:2392-2540).
+struct DiffArgs {
+    const double *uold;
+    double *unew;
+    const uint8_t *refined;
+    const int32_t *nbor;   // [noct][27] of the level
+    const int32_t *ckey;   // [ngrid][3]
+    int head, noct, ndim;
+    int boxmin[3], boxmax[3];
+    int refl;              // reflexive faces, bit 2*dim+side
+    const double *dt_dev;  // if non-null, dt is read from device memory
+    double dt, dx, difmag, smallr, g[3];
+    int reflux, ngridmax;
+    const int32_t *ghost_parent;
+    double *racc;          // as GodArgs::racc / rdet
+    int rdet;
+};
+hipError_t launch_diffusion3d(const DiffArgs &a, hipStream_t s);
+
 // Ordered brick-shape support list: returns true when (bx,by,bz) has a compiled kernel.
 bool brick_shape_supported(int bx, int by, int bz);
 

@@ -318,15 +318,16 @@ __device__ __forceinline__ void slope27(const T (*sq)[5][NP], int sm, int s0, in
 constexpr int kRefluxSlots = kRefluxSlotsPerGhost;
 __device__ __forceinline__ void reflux_add(double *unew, const int32_t *ghost_parent, int ngridmax, int vghost,
                                            double sign, const double F[5], int ttd, double *racc = nullptr,
-                                           int rdet = 0, int slot = 0)
+                                           int rdet = 0, int slot = 0, bool accumulate = false)
 {
     const int gi = vghost - ngridmax - 1;
     if (racc) {
         const double w = 1.0 / (double)ttd;
         if (rdet) {
+            // accumulate: the diffusion pass adds to the slot the Godunov launch wrote
+            double *q = racc + 5 * ((size_t)gi * kRefluxSlots + slot);
 #pragma unroll
-            for (int n = 0; n < 5; n++)
-                racc[5 * ((size_t)gi * kRefluxSlots + slot) + n] = sign * (F[n] * w);
+            for (int n = 0; n < 5; n++) q[n] = accumulate ? q[n] + sign * (F[n] * w) : sign * (F[n] * w);
             return;
         }
 #pragma unroll
@@ -1735,6 +1736,134 @@ hipError_t launch_godunov3d_wl(const WorkList &w, int bx, int by, int bz, int sl
     RH_BRICK_SHAPES(RH_LAUNCH_WL)
 #undef RH_LAUNCH_WL
     return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// Artificial diffusion (difmag > 0): cmpdivu + cmpdiff, hydro/umuscl.f90:
+// 2392-2540, called by unsplit at :273-283.  The diffusive flux of a face,
+// dt * difmag * min(0, mean of the velocity divergence at its 2^(ndim-1)
+// corner nodes) * (u_R - u_L), is linear in the update, so it runs as a pass
+// after the Godunov launch: unew += (D_low - D_high) per dimension, D zeroed
+// where either cell is refined (as the fluxes), refluxed into the parent cell
+// across coarse-fine faces (as godfine1 refluxes flux + D).  Each cell
+// computes its own faces; the two cells of a face compute the same D from the
+// same values in the same order.
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_diffusion3d(const DiffArgs a)
+{
+#pragma clang fp contract(off)
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)a.noct * 8) return;
+    const int o = a.head - 1 + (int)(i >> 3), ind = (int)(i & 7);
+    const int c0[3] = {ind & 1, (ind >> 1) & 1, (ind >> 2) & 1};
+    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    const double gdt[3] = {a.g[0] * (dt * 0.5), a.g[1] * (dt * 0.5), a.g[2] * (dt * 0.5)};
+    const int32_t *nb = a.nbor + (size_t)(o - (a.head - 1)) * 27;
+    // velocities (with ctoprim's gravity kick) of the 3x3x3 cells around the
+    // cell, the conservative states of the cell and its 6 face neighbours,
+    // the refined flags, and the oct index of the face neighbours
+    double q[3][3][3][3];
+    double uf[7][5];
+    uint8_t rf[7];
+    int vf[7];
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int cc[3] = {c0[0] + dx, c0[1] + dy, c0[2] + dz};
+                int oo[3], cb[3];
+                for (int d = 0; d < 3; d++) {
+                    oo[d] = cc[d] >> 1;  // -1, 0 or 1
+                    cb[d] = cc[d] & 1;
+                }
+                const int v = nb[(oo[0] + 1) + 3 * (oo[1] + 1) + 9 * (oo[2] + 1)];
+                const int okey[3] = {a.ckey[3 * o] + oo[0], a.ckey[3 * o + 1] + oo[1], a.ckey[3 * o + 2] + oo[2]};
+                const CellRef r = cell_ref3(a.uold, a.refined, v, okey, a.boxmin, a.boxmax, a.refl, cb);
+                double u[5];
+                uint8_t f;
+                cell_load3(r, u, f);
+                cell_flip(u, r.flip);
+                const double rho = u[0] > a.smallr ? u[0] : a.smallr;
+                const double ir = 1.0 / rho;
+                for (int d = 0; d < 3; d++) q[dz + 1][dy + 1][dx + 1][d] = u[1 + d] * ir + gdt[d];
+                const int nz = (dx != 0) + (dy != 0) + (dz != 0);
+                if (nz <= 1) {
+                    // 0: the cell; 1..6: -x +x -y +y -z +z
+                    const int k = nz == 0 ? 0 : dx ? (dx < 0 ? 1 : 2) : dy ? (dy < 0 ? 3 : 4) : (dz < 0 ? 5 : 6);
+                    for (int n = 0; n < 5; n++) uf[k][n] = u[n];
+                    rf[k] = f;
+                    vf[k] = v;
+                }
+            }
+    // divergence at the node (x, y, z) in {0,1}^3 of the cell's corners,
+    // cmpdivu's expression order (node (i,j,k) uses cells i-1..i, j-1..j, k-1..k)
+    const double fact = 0.25 / a.dx;  // half**(ndim-1)/dx
+    auto Q = [&](int x, int y, int z, int d) { return q[z + 1][y + 1][x + 1][d]; };
+    double dv[2][2][2];
+    for (int z = 0; z < 2; z++)
+        for (int y = 0; y < 2; y++)
+            for (int x = 0; x < 2; x++) {
+                double ux = 0.0, vy = 0.0, wz = 0.0;
+                ux = ux + fact * (Q(x, y, z, 0) - Q(x - 1, y, z, 0));
+                ux = ux + fact * (Q(x, y - 1, z, 0) - Q(x - 1, y - 1, z, 0));
+                vy = vy + fact * (Q(x, y, z, 1) - Q(x, y - 1, z, 1) + Q(x - 1, y, z, 1) - Q(x - 1, y - 1, z, 1));
+                ux = ux + fact * (Q(x, y, z - 1, 0) - Q(x - 1, y, z - 1, 0) + Q(x, y - 1, z - 1, 0) -
+                                  Q(x - 1, y - 1, z - 1, 0));
+                vy = vy + fact * (Q(x, y, z - 1, 1) - Q(x, y - 1, z - 1, 1) + Q(x - 1, y, z - 1, 1) -
+                                  Q(x - 1, y - 1, z - 1, 1));
+                wz = wz + fact * (Q(x, y, z, 2) - Q(x, y, z - 1, 2) + Q(x, y - 1, z, 2) - Q(x, y - 1, z - 1, 2) +
+                                  Q(x - 1, y, z, 2) - Q(x - 1, y, z - 1, 2) + Q(x - 1, y - 1, z, 2) -
+                                  Q(x - 1, y - 1, z - 1, 2));
+                dv[z][y][x] = ux + vy + wz;
+            }
+    const double f4 = 0.25;  // half**(ndim-1)
+    const bool myref = a.refined[(size_t)o * 8 + ind] != 0;
+    double un[5];
+    for (int n = 0; n < 5; n++) un[n] = a.unew[(size_t)o * 40 + n * 8 + ind];
+    for (int d = 0; d < 3; d++) {
+        double D[2][5];
+        for (int side = 0; side < 2; side++) {
+            // face at the cell's low (side 0) or high (side 1) boundary of dimension d
+            double div1;
+            if (d == 0) {
+                div1 = f4 * dv[0][0][side];
+                div1 = div1 + f4 * dv[0][1][side];
+                div1 = div1 + f4 * (dv[1][0][side] + dv[1][1][side]);
+            } else if (d == 1) {
+                div1 = 0.0;
+                div1 = div1 + f4 * (dv[0][side][0] + dv[0][side][1]);
+                div1 = div1 + f4 * (dv[1][side][0] + dv[1][side][1]);
+            } else {
+                div1 = f4 * (dv[side][0][0] + dv[side][0][1] + dv[side][1][0] + dv[side][1][1]);
+            }
+            div1 = a.difmag * (div1 < 0.0 ? div1 : 0.0);
+            const int k = 1 + 2 * d + side;
+            const bool z = myref || rf[k];
+            for (int n = 0; n < 5; n++) {
+                const double du = side == 0 ? uf[0][n] - uf[k][n] : uf[k][n] - uf[0][n];
+                D[side][n] = z ? 0.0 : dt * div1 * du;
+            }
+            // coarse-fine face: the neighbour is an interpolated ghost oct
+            if (a.reflux && !z && vf[k] > a.ngridmax) {
+                const int sub = d == 0 ? ((ind >> 1) & 1) + 2 * ((ind >> 2) & 1)
+                              : d == 1 ? (ind & 1) + 2 * ((ind >> 2) & 1)
+                                       : (ind & 1) + 2 * ((ind >> 1) & 1);
+                // ghost on the low side sends -D/8 (its high face slot), on the high side +D/8
+                reflux_add(a.unew, a.ghost_parent, a.ngridmax, vf[k], side == 0 ? -1.0 : 1.0, D[side], 8, a.racc,
+                           a.rdet, side == 0 ? 4 * (2 * d + 1) + sub : 4 * (2 * d) + sub, a.rdet != 0);
+            }
+        }
+        for (int n = 0; n < 5; n++) un[n] = un[n] + (D[0][n] - D[1][n]);
+    }
+    for (int n = 0; n < 5; n++) a.unew[(size_t)o * 40 + n * 8 + ind] = un[n];
+}
+
+hipError_t launch_diffusion3d(const DiffArgs &a, hipStream_t s)
+{
+    const long n = (long)a.noct * 8;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_diffusion3d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_godunov1d(const God1Args &a, int prec, hipStream_t s)

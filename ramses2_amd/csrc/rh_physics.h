@@ -246,22 +246,23 @@ __device__ __forceinline__ void riemann_hllc(const T ql[5], const T qr[5], T f[5
     T inv_den = trcp(rcr + rcl);
     T ustar = (rcr * ur + rcl * ul + (Pl - Pr)) * inv_den;
     T Ptotstar = (rcr * Pl + rcl * Pr + rcl * rcr * (ul - ur)) * inv_den;
-    bool lstar = ustar > T(0);
-    T S = lstar ? SL : SR;
-    T rs = lstar ? rl : rr, us = lstar ? ul : ur, Ps = lstar ? Pl : Pr, es = lstar ? etotl : etotr;
+    // The reference samples, in this order: SL > 0 -> left state; ustar > 0
+    // -> left star state; SR > 0 -> right star state; else right state.  The
+    // side whose values are used is left exactly when SL > 0 or ustar > 0,
+    // and the star state is taken when SL <= 0 and (ustar > 0 or SR > 0); in
+    // both star cases the side is the reference's ustar > 0.  So one select
+    // of the side's seven values and one of four values (state or star) give
+    // the reference's result (its chain costs three selects per value).
+    const bool sl_pos = SL > T(0), us_pos = ustar > T(0);
+    const bool left = sl_pos || us_pos;
+    const bool star = !sl_pos && (us_pos || SR > T(0));
+    T S = left ? SL : SR;
+    T rs = left ? rl : rr, us = left ? ul : ur, Ps = left ? Pl : Pr, es = left ? etotl : etotr;
+    T vo = left ? vl : vr, wo = left ? wl : wr;
     T inv_sd = trcp(S - ustar);
     T rstar = rs * (S - us) * inv_sd;
     T etotstar = ((S - us) * es - Ps * us + Ptotstar * ustar) * inv_sd;
-    T ro, uo, vo, wo, Po, eo;
-    if (SL > T(0)) {
-        ro = rl; uo = ul; vo = vl; wo = wl; Po = Pl; eo = etotl;
-    } else if (lstar) {
-        ro = rstar; uo = ustar; vo = vl; wo = wl; Po = Ptotstar; eo = etotstar;
-    } else if (SR > T(0)) {
-        ro = rstar; uo = ustar; vo = vr; wo = wr; Po = Ptotstar; eo = etotstar;
-    } else {
-        ro = rr; uo = ur; vo = vr; wo = wr; Po = Pr; eo = etotr;
-    }
+    T ro = star ? rstar : rs, uo = star ? ustar : us, Po = star ? Ptotstar : Ps, eo = star ? etotstar : es;
     f[0] = ro * uo;
     f[1] = ro * uo * uo + Po;
     f[2] = ro * uo * vo;

@@ -298,6 +298,7 @@ int rh_job_prepare(int head_idx, int num_octs, int ilevel, double dx, double gam
     State &s = st();
     if (!s.inited || !s.uold) return efail(RH_ERR_STATE, "library not initialised / mesh not allocated");
     if (s.ndim != 3) return efail(RH_ERR_ARG, "batched jobs are 3D");
+    if (s.difmag > 0.0) return efail(RH_ERR_ARG, "difmag > 0 runs through rh_godunov and rh_level_run, not batched jobs");
     GodArgs a;
     const LevelBricks *lb = nullptr;
     if (int r = fill_args3(head_idx, num_octs, ilevel, dx, gamma, smallr, smallc2, courant_factor, g, slope,

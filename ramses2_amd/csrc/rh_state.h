@@ -137,6 +137,9 @@ struct State {
     // writes its own slot, folded in a fixed order; racc_det holds the slots
     // of the per-level drop-in calls
     int deterministic = 0;
+    // artificial diffusion coefficient (hydro_params_t%difmag); > 0 adds the
+    // cmpdiff pass after each 3D Godunov launch
+    double difmag = 0.0;
     double *racc_det = nullptr;
     long racc_det_cap = 0;            // ghosts it has room for
     int interpol_var = 0, interpol_type = 1;
@@ -167,6 +170,11 @@ int fill_args1(int head_idx, int num_octs, int ilevel, double dx, double gamma, 
                God1Args &a);
 // Grow the current context's per-brick partial-sum buffer to n entries.
 int ensure_partials(int n);
+// The artificial-diffusion pass of a 3D level after its Godunov launch
+// (unew += cmpdiff fluxes of uold), when difmag > 0.
+hipError_t enqueue_diffusion(State &s, int ilevel, int head_idx, int num_octs, double dx, double smallr,
+                             const double g[3], const double *dt_dev, double dt, int reflux, double *racc,
+                             int rdet);
 // Destroy the context's cached rh_level_run graph (its arrays or tables change).
 void drop_level_graph(State &s);
 

@@ -29,6 +29,7 @@ class HydroParams:
     smallc: float = 1e-10
     slope_type: int = 1
     riemann: str = "llf"
+    difmag: float = 0.0   # artificial diffusion (3D; rh_set_difmag)
 
     @property
     def riemann_id(self):
@@ -138,6 +139,7 @@ class HipHydro:
                 precision=0):
         g = np.ascontiguousarray(gravity, dtype=np.float64)
         call("rh_set_precision", int(precision))
+        call("rh_set_difmag", float(p.difmag))
         smax_cap = p.courant_factor * dx / dt
         call("rh_godunov", head, num, self.ngridmax, ilevel, levelmin, levelmax, p.gamma, p.smallr,
              p.smallc ** 2, dt, dx, smax_cap, p.slope_type, p.riemann_id, ptr(g))
@@ -227,6 +229,7 @@ class UniformLevel:
         g = np.ascontiguousarray(gravity, dtype=np.float64)
         call("rh_set_precision", int(precision))
         call("rh_set_fused_courant", int(fused_courant))
+        call("rh_set_difmag", float(self.p.difmag))
         dt_last = ctypes.c_double()
         t_tot = ctypes.c_double()
         sums = np.zeros(4)

@@ -317,9 +317,11 @@ class EnsembleServer:
             # admissions overlap the tick on the device
             while queue and len(active) < self.max_slots:
                 need = eng.footprint(queue[0])
-                if need > self.hbm_budget:
-                    raise ValueError("job %d needs %d bytes, more than the HBM budget" % (queue[0].job_id, need))
-                if in_use + need > self.hbm_budget:
+                reserved = eng.template_bytes()  # the per-size template meshes, resident beside the jobs
+                if reserved + need > self.hbm_budget:
+                    raise ValueError("job %d needs %d bytes, more than the HBM budget beside the %d bytes of "
+                                     "templates" % (queue[0].job_id, need, reserved))
+                if reserved + in_use + need > self.hbm_budget:
                     break
                 spec = queue.popleft()
                 h = eng.open(spec)

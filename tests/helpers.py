@@ -21,7 +21,7 @@ def oracle_uniform(mesh: UniformMesh, uold, periodic=True, bound_type=2, unew=No
 
 def oracle_params(p):
     return O.make_params(gamma=p.gamma, smallr=p.smallr, smallc=p.smallc, courant_factor=p.courant_factor,
-                         slope_type=p.slope_type, riemann=p.riemann_id)
+                         slope_type=p.slope_type, riemann=p.riemann_id, difmag=p.difmag)
 
 
 def rel_err_per_var(a, b):

@@ -64,9 +64,13 @@ class CountingEngine:
         self.max_bytes = 0
         self.order = []
         self.next = 1
+        self.templates = 0
 
     def footprint(self, spec):
         return 1000 * spec.ncell ** 3
+
+    def template_bytes(self):
+        return self.templates
 
     def open(self, spec):
         h = self.next
@@ -137,3 +141,17 @@ def test_closed_batch_admits_everything_at_once():
     res = srv.run(specs, realtime=False)
     assert eng.max_open == 10 and srv.ticks == 3
     assert all(r.status["steps"] == 3 for r in res.values())
+
+
+def test_admission_counts_the_resident_templates():
+    """The per-size template meshes stay on the device beside the jobs, so
+    admission keeps jobs + templates within the HBM budget."""
+    specs = make_stream(40, rate=400.0, seed=6, nmin=8, nmax=32, nsteps=5)
+    clock = FakeClock()
+    eng = CountingEngine(clock)
+    eng.templates = 1000 * 32 ** 3
+    budget = 1000 * 3 * 32 ** 3
+    srv = EnsembleServer(eng, max_slots=8, hbm_budget=budget, clock=clock, sleep=clock.sleep)
+    res = srv.run(specs)
+    assert sorted(res) == [s.job_id for s in specs]
+    assert eng.max_bytes + eng.templates <= budget
