@@ -198,3 +198,30 @@ def test_deterministic_reflux_is_repeatable(path):
     for _ in range(3):
         oracle_step(am, om, p)
     assert np.all(rel_err_per_var(states[0], om.uold) < 1e-12)
+
+
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_two_level_diffusion(deterministic):
+    """difmag = 0.1 on a two-level mesh through the per-level calls: the
+    diffusion pass's fluxes are refluxed into the parent cells with the
+    Godunov fluxes (atomically, or into the per-face slots)."""
+    from ramses2_amd._lib import call
+    am = two_level_mesh()
+    u = smooth_state(am, 17, blast=True)
+    om = oracle_mesh(am, u, 0, 1)
+    restrict_all(am, om)
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc", difmag=0.1)
+    print("case", deterministic, flush=True)
+    try:
+        gpu = AmrHierarchy(am, om.uold.copy(), p, deterministic=deterministic)
+        for step in range(2):
+            dt_o = oracle_step(am, om, p)
+            dt_g = gpu.step()
+            assert abs(dt_g - dt_o) <= 1e-14 * dt_o
+            assert np.all(rel_err_per_var(gpu.state(), om.uold) < 1e-12)
+        u0 = gpu.state()
+    finally:
+        call("rh_set_difmag", 0.0)
+        call("rh_set_deterministic", 0)
+    mass = [leaf_total(am, x, 0) for x in (om.uold, u0)]
+    assert abs(mass[1] - mass[0]) <= 1e-13 * abs(mass[0])

@@ -250,3 +250,20 @@ def test_hydro_flag_matches_reference(mesh, crit):
         nflag += n
     # the criterion flags some cells and leaves others
     assert 0 < nflag < sum(om.noct) * 8
+
+
+def test_two_level_diffusion_matches_reference():
+    """Artificial diffusion (difmag = 0.1) across a coarse-fine boundary: the
+    diffusive fluxes are refluxed with the Godunov fluxes."""
+    am = two_level_mesh()
+    u = smooth_state(am, 17, blast=True)
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
+    po = oracle_params(p)
+    po.difmag = 0.1
+    om = oracle_mesh(am, u, 0, 1)
+    ref = O.RefLevel(om, po)
+    restrict_both(am, om, ref)
+    for _ in range(2):
+        _, co, cr = step_both(om, ref, am.levels, po, p.smallc, am.dx)
+        assert np.array_equal(co, cr)
+        assert_same(om, ref)
