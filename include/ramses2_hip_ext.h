@@ -57,6 +57,15 @@ int rh_get_brick_shape(int32_t *bx, int32_t *by, int32_t *bz);
  * copy happens when capacity >= *len (else RH_ERR_ARG). */
 int rh_get_level_table(int ilevel, int which, int32_t *out, int64_t capacity, int64_t *len);
 
+/* The brick lattice rh_build_nbor gives a 3D level whose octs have these keys
+ * ([noct][3], each >= box_min): bricks of bx*by*bz octs with origins at
+ * box_min + off + multiples of the shape, off chosen per level to minimise
+ * the modelled march work, returned in *work (per brick: 0.35(2bx+4)(2by+4)
+ * P cells and (2bx+2)(2by+2) columns per marched plane).  Host only: no
+ * device call; the autotuner ranks brick shapes with it. */
+int rh_brick_lattice(const int32_t *ckey, int noct, const int32_t box_min[3], const int32_t box_max[3], int bx,
+                     int by, int bz, int32_t off[3], double *work);
+
 /* HIP-event timer on the library stream: milliseconds between start and stop. */
 int rh_timer_start(void);
 int rh_timer_stop(float *ms);

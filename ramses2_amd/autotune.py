@@ -126,23 +126,31 @@ def tune_amr(amesh, uold, params, precision=0, steps=4, shapes=None, cache=DEFAU
     return bricks, times
 
 
-def level_work(keys, shape, box_min=(0, 0, 0)):
-    """Modelled work of one level's bricks: a brick on the grid rh_build_nbor
-    uses (origins at box_min + multiples of the shape) marches the planes of
-    its occupied z layers plus two halo planes each side; per plane the P phase
-    converts (2BX+4)(2BY+4) cells and the T/F/U phases run (2BX+2)(2BY+2)
-    columns, a P cell costing about a third of a T/F/U column-plane."""
-    bx, by, bz = shape
-    k = np.asarray(keys, dtype=np.int64) - np.asarray(box_min)
-    b = k // np.asarray(shape)
-    key = b[:, 0] | (b[:, 1] << 21) | (b[:, 2] << 42)
-    order = np.argsort(key, kind="stable")
-    key, z = key[order], (k[:, 2] - b[:, 2] * bz)[order]
-    _, first = np.unique(key, return_index=True)
-    planes = 2 * (np.maximum.reduceat(z, first) - np.minimum.reduceat(z, first) + 1)
-    p_cells = (2 * bx + 4) * (2 * by + 4)
-    columns = (2 * bx + 2) * (2 * by + 2)
-    return float(np.sum(0.35 * p_cells * (planes + 4) + columns * (planes + 2)))
+def brick_offset(keys, shape, box_min=(0, 0, 0), box_max=None):
+    """The lattice offset rh_build_nbor gives a level's bricks of this shape
+    (origins at box_min + offset + multiples of the shape) and its modelled
+    work, from the library's own choice (rh_brick_lattice, host only).  With
+    no box_max the level is taken as sparse in an unbounded box."""
+    k = np.ascontiguousarray(np.asarray(keys, dtype=np.int32).reshape(-1, 3))
+    bmin = np.ascontiguousarray(np.asarray(box_min, dtype=np.int32)[:3])
+    if box_max is None:
+        box_max = np.full(3, 1 << 30, dtype=np.int32)
+    bmax = np.ascontiguousarray(np.asarray(box_max, dtype=np.int32)[:3])
+    off = np.zeros(3, dtype=np.int32)
+    work = ctypes.c_double()
+    ip = ctypes.POINTER(ctypes.c_int32)
+    call("rh_brick_lattice", k.ctypes.data_as(ip), len(k), bmin.ctypes.data_as(ip), bmax.ctypes.data_as(ip),
+         int(shape[0]), int(shape[1]), int(shape[2]), off.ctypes.data_as(ip), ctypes.byref(work))
+    return tuple(int(x) for x in off), work.value
+
+
+def level_work(keys, shape, box_min=(0, 0, 0), box_max=None):
+    """Modelled work of one level's bricks on the lattice rh_build_nbor picks
+    (brick_offset): a brick marches the planes of its occupied z layers plus
+    two halo planes each side; per plane the P phase converts (2BX+4)(2BY+4)
+    cells and the T/F/U phases run (2BX+2)(2BY+2) columns, a P cell costing
+    about a third of a T/F/U column-plane (0.35)."""
+    return brick_offset(keys, shape, box_min, box_max)[1]
 
 
 def count_bricks(keys, shape, box_min=(0, 0, 0)):
@@ -154,11 +162,12 @@ def count_bricks(keys, shape, box_min=(0, 0, 0)):
 def level_shapes(amesh, shapes=None):
     """Per level, the shape with the least modelled work (level_work)."""
     shapes = shapes or [s for s in SHAPES if s != (1, 1, 1)]
+    _, _, box_min, box_max = amesh.level_params()
     out = {}
     for l in amesh.levels:
         i0 = amesh.head[l] - 1
         k = amesh.ckey[i0:i0 + amesh.noct[l]]
-        out[l] = min(shapes, key=lambda s: level_work(k, s))
+        out[l] = min(shapes, key=lambda s: level_work(k, s, box_min[l - 1], box_max[l - 1]))
     return out
 
 

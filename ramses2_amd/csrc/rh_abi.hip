@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <climits>
 #include <map>
 #include <string>
 #include <vector>
@@ -362,6 +363,78 @@ static ResolveCtx make_resolve(int ilevel, const int32_t box_min[3], const int32
         c.periodic[d] = d < s.ndim ? periodic[d] : 1;
     }
     return c;
+}
+
+// Lattice offset of a level's bricks.  The octs of a refined region come in
+// the 2x2x2 blocks of the parent octs they refine, so where a level is sparse
+// the number of bricks that cover it, and the planes they march, depend on
+// where the brick lattice starts.  Every offset in [0,B) per dimension that
+// keeps all origins inside the box (no oct below boxmin + offset) is scored
+// by the modelled work of its bricks: per brick, the P phase converts
+// (2BX+4)(2BY+4) cells and the T/F/U phases run (2BX+2)(2BY+2) columns on
+// each marched plane (the occupied z layers plus the halo planes), a P cell
+// costing about a third of a column-plane.  Levels filling at least half
+// their box keep offset 0; large searches try only the aligned and
+// half-shifted lattices.  Returns the modelled work of the chosen lattice.
+static double choose_brick_offset(const int32_t *key, int n, const int32_t boxmin[3], const int32_t boxmax[3],
+                                  const int B[3], int off[3])
+{
+    off[0] = off[1] = off[2] = 0;
+    if (n <= 0) return 0.0;
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], key[3 * i + d]);
+            hi[d] = std::max(hi[d], key[3 * i + d]);
+        }
+    double vol = 1;
+    for (int d = 0; d < 3; d++) vol *= (double)boxmax[d] - (double)boxmin[d];
+    const bool dense = 2.0 * n > vol;
+    std::vector<int> cand[3];
+    double ncand = 1;
+    for (int d = 0; d < 3; d++) {
+        for (int o = 0; o < (dense ? 1 : B[d]); o++)
+            if (o == 0 || o <= lo[d] - boxmin[d]) cand[d].push_back(o);
+        ncand *= (double)cand[d].size();
+    }
+    if (ncand * n > 6e7)
+        for (int d = 0; d < 3; d++) {
+            std::vector<int> c2{0};
+            if (B[d] > 1 && B[d] / 2 <= lo[d] - boxmin[d]) c2.push_back(B[d] / 2);
+            cand[d] = c2;
+        }
+    const double pcell = 0.35 * (2 * B[0] + 4) * (2 * B[1] + 4), col = (double)(2 * B[0] + 2) * (2 * B[1] + 2);
+    std::vector<int32_t> occ;
+    double best = -1;
+    for (int oz : cand[2])
+        for (int oy : cand[1])
+            for (int ox : cand[0]) {
+                const int o[3] = {ox, oy, oz};
+                int g0[3], ng[3];
+                for (int d = 0; d < 3; d++) {
+                    g0[d] = (lo[d] - boxmin[d] - o[d]) / B[d];
+                    ng[d] = (hi[d] - boxmin[d] - o[d]) / B[d] - g0[d] + 1;
+                }
+                occ.assign((size_t)ng[0] * ng[1] * ng[2], -1);
+                for (int i = 0; i < n; i++) {
+                    int b[3];
+                    for (int d = 0; d < 3; d++) b[d] = (key[3 * i + d] - boxmin[d] - o[d]) / B[d] - g0[d];
+                    const int z = (key[3 * i + 2] - boxmin[2] - o[2]) % B[2];
+                    int32_t &v = occ[((size_t)b[2] * ng[1] + b[1]) * ng[0] + b[0]];
+                    v = v < 0 ? (z | z << 16) : (std::min(v & 0xffff, z) | std::max(v >> 16, z) << 16);
+                }
+                double cost = 0;
+                for (int32_t v : occ)
+                    if (v >= 0) {
+                        const double planes = 2.0 * ((v >> 16) - (v & 0xffff) + 1);
+                        cost += pcell * (planes + 4) + col * (planes + 2);
+                    }
+                if (best < 0 || cost < best) {
+                    best = cost;
+                    off[0] = ox, off[1] = oy, off[2] = oz;
+                }
+            }
+    return best;
 }
 
 }  // namespace rh
@@ -799,19 +872,23 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
         lb.by = s.brick_by;
         lb.bz = s.brick_bz;
         lb.bnd = !(c.periodic[0] && c.periodic[1] && c.periodic[2]);
-        // enumerate the bricks that hold at least one oct of the level
+        // enumerate the bricks that hold at least one oct of the level, on
+        // the lattice boxmin + off + multiples of the shape
+        const int B[3] = {lb.bx, lb.by, lb.bz};
+        choose_brick_offset(s.h_ckey.data() + 3 * (size_t)(head_idx - 1), num_octs, c.boxmin, c.boxmax, B, lb.off);
+        const int32_t lmin[3] = {c.boxmin[0] + lb.off[0], c.boxmin[1] + lb.off[1], c.boxmin[2] + lb.off[2]};
         std::vector<int64_t> keys;
-        const int64_t nbx = (c.boxmax[0] - c.boxmin[0] + lb.bx - 1) / lb.bx;
-        const int64_t nby = (c.boxmax[1] - c.boxmin[1] + lb.by - 1) / lb.by;
-        const int64_t nbz = (c.boxmax[2] - c.boxmin[2] + lb.bz - 1) / lb.bz;
+        const int64_t nbx = (c.boxmax[0] - lmin[0] + lb.bx - 1) / lb.bx;
+        const int64_t nby = (c.boxmax[1] - lmin[1] + lb.by - 1) / lb.by;
+        const int64_t nbz = (c.boxmax[2] - lmin[2] + lb.bz - 1) / lb.bz;
         auto brick_of = [&](size_t i) {
-            const int64_t bx = (s.h_ckey[3 * i] - c.boxmin[0]) / lb.bx;
-            const int64_t by = (s.h_ckey[3 * i + 1] - c.boxmin[1]) / lb.by;
-            const int64_t bz = (s.h_ckey[3 * i + 2] - c.boxmin[2]) / lb.bz;
+            const int64_t bx = (s.h_ckey[3 * i] - lmin[0]) / lb.bx;
+            const int64_t by = (s.h_ckey[3 * i + 1] - lmin[1]) / lb.by;
+            const int64_t bz = (s.h_ckey[3 * i + 2] - lmin[2]) / lb.bz;
             return (bz * nby + by) * nbx + bx;
         };
         std::vector<int32_t> zr;  // per brick: z layers holding octs, zlo | zhi << 16
-        auto zlayer = [&](size_t i) { return (s.h_ckey[3 * i + 2] - c.boxmin[2]) % lb.bz; };
+        auto zlayer = [&](size_t i) { return (s.h_ckey[3 * i + 2] - lmin[2]) % lb.bz; };
         if (nbx * nby * nbz <= (int64_t(1) << 26)) {
             // occupancy map of the level's brick grid: linear in octs + bricks
             std::vector<int32_t> occ((size_t)(nbx * nby * nbz), -1);
@@ -848,9 +925,9 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
         std::vector<int32_t> org(3 * keys.size());
         for (size_t b = 0; b < keys.size(); b++) {
             const int64_t k = keys[b];
-            org[3 * b + 0] = c.boxmin[0] + (int32_t)(k % nbx) * lb.bx;
-            org[3 * b + 1] = c.boxmin[1] + (int32_t)((k / nbx) % nby) * lb.by;
-            org[3 * b + 2] = c.boxmin[2] + (int32_t)(k / (nbx * nby)) * lb.bz;
+            org[3 * b + 0] = lmin[0] + (int32_t)(k % nbx) * lb.bx;
+            org[3 * b + 1] = lmin[1] + (int32_t)((k / nbx) % nby) * lb.by;
+            org[3 * b + 2] = lmin[2] + (int32_t)(k / (nbx * nby)) * lb.bz;
         }
         const int nw = (lb.bx + 2) * (lb.by + 2) * (lb.bz + 2);
         RH_HIP_CHECK(dmalloc((void **)&lb.org, org.size() * sizeof(int32_t)));
@@ -1170,6 +1247,21 @@ int rh_get_level_table(int ilevel, int which, int32_t *out, int64_t capacity, in
         RH_HIP_CHECK(hipMemcpyAsync(out, src, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, s.stream));
         RH_HIP_CHECK(hipStreamSynchronize(s.stream));
     }
+    return 0;
+}
+
+int rh_brick_lattice(const int32_t *ckey, int noct, const int32_t box_min[3], const int32_t box_max[3], int bx,
+                     int by, int bz, int32_t off[3], double *work)
+{
+    if (noct < 0 || (noct > 0 && !ckey) || !box_min || !box_max || !off || bx < 1 || by < 1 || bz < 1)
+        return fail(RH_ERR_ARG, "rh_brick_lattice: bad arguments");
+    for (int i = 0; i < 3 * noct; i++)
+        if (ckey[i] < box_min[i % 3]) return fail(RH_ERR_ARG, "rh_brick_lattice: oct key below box_min");
+    const int B[3] = {bx, by, bz};
+    int o[3];
+    const double w = choose_brick_offset(ckey, noct, box_min, box_max, B, o);
+    for (int d = 0; d < 3; d++) off[d] = o[d];
+    if (work) *work = w;
     return 0;
 }
 

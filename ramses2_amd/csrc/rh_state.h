@@ -38,6 +38,7 @@ struct LevelBricks {
     int32_t *zr = nullptr;    // [nbricks] z layers holding octs of the level: zlo | zhi << 16
     int any_ref = 1;          // some cell of the level's octs is refined
     int bnd = 1;              // some dimension is not periodic (boundary ghosts in the windows)
+    int off[3] = {0, 0, 0};   // brick lattice offset: origins at boxmin + off + multiples of the shape
     int boxmin[3] = {0, 0, 0};
     int boxmax[3] = {0, 0, 0};
 };

@@ -22,6 +22,7 @@ import numpy as np
 import pytest
 
 from ramses2_amd import HydroParams, UniformLevel, UniformMesh, ic
+from ramses2_amd.autotune import brick_offset
 from ramses2_amd.hydro import AmrHierarchy
 from tests.amr_cases import low_face_mesh, smooth_state, three_level_mesh, two_level_mesh
 
@@ -118,6 +119,14 @@ def check_level(hh, orc, l, keys, head, levelmin, bricks=True):
     if bricks and ndim == 3:
         bx, by, bz = hh.brick_shape()
         org = hh.level_table(l, 2).reshape(-1, 3)
+        # one lattice per level, at the offset the library's work model picks,
+        # with every origin inside the box and one brick per occupied cell of it
+        bmin = np.array([int(x) for x in orc.box_min[l - 1][:3]])
+        off, _ = brick_offset(keys, (bx, by, bz), bmin, [int(x) for x in orc.box_max[l - 1][:3]])
+        assert np.all(org >= bmin)
+        assert np.all((org - bmin - np.array(off)) % np.array([bx, by, bz]) == 0), (off, org[:4])
+        cells = (np.asarray(keys, dtype=np.int64)[:, :3] - bmin - np.array(off)) // np.array([bx, by, bz])
+        assert len(org) == len(np.unique(cells, axis=0))
         win = hh.level_table(l, 3).reshape(len(org), bz + 2, by + 2, bx + 2)
         for b, o in enumerate(org):
             for z in range(bz + 2):
