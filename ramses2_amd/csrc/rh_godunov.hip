@@ -427,7 +427,12 @@ __device__ __forceinline__ int ring_slot3(int t) { return (t + 3) % 3; }
 // A is GodArgs (a kernel argument) or GodArgs in the constant address space
 // (a work-list descriptor): both are read-only for the kernel's lifetime, so
 // the compiler reloads their fields with scalar loads instead of holding them.
-template <int BX, int BY, int BZ, typename T, int RS, int NS, bool WL, typename A>
+// GEN = false compiles the body for a level that is periodic in every
+// dimension, has no refined cell and no coarse-fine faces (every uniform
+// level): without the boundary-ghost gather, the refined-face zeroing and the
+// reflux code, the FP64 kernel needs 224 instead of 245 VGPRs and no SGPR
+// spills (39 with them), and runs 2-3% faster.  The launch checks the level.
+template <int BX, int BY, int BZ, typename T, int RS, int NS, bool WL, bool GEN = true, typename A>
 __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY, BZ, T, NS, sep_flux<BX, BY, BZ, T, NS>()> &sm,
                                            const int part = -1, const int pslot = -1)
 {
@@ -479,7 +484,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     auto fscale = [&](double f) -> double { return sizeof(T) == 4 ? f * dtdx64 : (f * dt) * inv_dx; };
     const int boxmin[3] = {a.boxmin[0], a.boxmin[1], a.boxmin[2]};
     const int boxmax[3] = {a.boxmax[0], a.boxmax[1], a.boxmax[2]};
-    const bool anyref = a.any_ref != 0;  // refined flags are all 0 when not
+    const bool anyref = GEN && a.any_ref != 0;  // refined flags are all 0 when not
     int refl = 0;
 #pragma unroll
     for (int d = 0; d < 3; d++)
@@ -580,7 +585,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         }
         ring_i = (ry + 2) * PX + (rx + 2);
     }
-    const bool bnd = RH_FAST_FETCH ? a.bnd != 0 : true;
+    const bool bnd = GEN && (RH_FAST_FETCH ? a.bnd != 0 : true);
     auto fetch = [&](int i, int tz, double u[5], uint8_t &rf, int &fl) {
         const int px = i % PX - 2, py = i / PX - 2;
         const int ox = (px + 2) >> 1, oy = (py + 2) >> 1, oz = (tz + 2) >> 1;
@@ -828,7 +833,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
                 for (int n = 0; n < 5; n++) fz[n] = z ? T(0) : fz[n];
             }
         }
-        if (a.reflux) {
+        if (GEN && a.reflux) {
             // window value and ownership of a cell of the trace region
             auto wv = [&](int x, int y, int z) { return swin[(((z + 2) >> 1) * WY + ((y + 2) >> 1)) * WX + ((x + 2) >> 1)]; };
             auto owned = [&](int x, int y, int z, int v) {
@@ -1016,7 +1021,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
 #define RH_BRICK_MIN_WAVES 2
 #endif
 
-template <int BX, int BY, int BZ, typename T, int RS, int NS>
+template <int BX, int BY, int BZ, typename T, int RS, int NS, bool GEN = true>
 __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2),
                                   (sizeof(T) == 8                              ? RH_BRICK_MIN_WAVES
                                    : f32_three_waves<BX, BY, BZ, T, false>() ? 3
@@ -1053,7 +1058,7 @@ __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2),
         }
     }
     if (bid >= a.nbricks) return;
-    brick_body<BX, BY, BZ, T, RS, NS, false>(a, bid, sm, part, pslot);
+    brick_body<BX, BY, BZ, T, RS, NS, false, GEN>(a, bid, sm, part, pslot);
 }
 
 // Work-list kernel: work item i = (descriptor, brick), where a descriptor
@@ -1678,6 +1683,9 @@ hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, 
 {
     if (nparts) *nparts = a.nbricks > 0 ? a.nbricks : 0;
     if (a.nbricks <= 0) return hipSuccess;
+    // the specialised body: periodic in every dimension, no refined cells,
+    // no refluxes
+    const bool gen = a.bnd || a.any_ref || a.reflux;
 #define RH_LAUNCH_SHAPE(A, B, C)                                                                         \
     if (bx == A && by == B && bz == C) {                                                                 \
         constexpr int nt = (2 * A + 2) * (2 * B + 2);                                                     \
@@ -1686,6 +1694,11 @@ hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, 
                 launch_brick(k_godunov3d<A, B, C, double, 0, 3>, nt, a, s, e0, e1, nparts);              \
             else                                                                                         \
                 launch_brick(k_godunov3d<A, B, C, float, 0, 3>, nt, a, s, e0, e1, nparts);               \
+        } else if (a.riemann == 3 && !gen) {                                                             \
+            if (prec == 0)                                                                               \
+                launch_brick(k_godunov3d<A, B, C, double, 3, 2, false>, nt, a, s, e0, e1, nparts);       \
+            else                                                                                         \
+                launch_brick(k_godunov3d<A, B, C, float, 3, 2, false>, nt, a, s, e0, e1, nparts);        \
         } else if (a.riemann == 3) {                                                                     \
             if (prec == 0)                                                                               \
                 launch_brick(k_godunov3d<A, B, C, double, 3, 2>, nt, a, s, e0, e1, nparts);              \
