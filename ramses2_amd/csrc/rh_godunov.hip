@@ -342,6 +342,24 @@ __device__ __forceinline__ void reflux_add(double *unew, const int32_t *ghost_pa
     for (int n = 0; n < 5; n++) atomicAdd(p + n * ttd, sign * (F[n] * w));
 }
 
+// The summed refluxes q (already weighted by 1/2^ndim, with their signs) of
+// one ghost oct: into its accumulator, or into its parent cell of unew.
+__device__ __forceinline__ void reflux_flush(double *unew, const int32_t *ghost_parent, int ngridmax, int vghost,
+                                             const double q[5], int ttd, double *racc)
+{
+    const int gi = vghost - ngridmax - 1;
+    if (racc) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) atomicAdd(racc + 5 * (size_t)gi + n, q[n]);
+        return;
+    }
+    const int igrid = ghost_parent[2 * gi], icell = ghost_parent[2 * gi + 1];
+    if (igrid <= 0) return;
+    double *p = unew + (size_t)(igrid - 1) * 5 * ttd + (icell - 1);
+#pragma unroll
+    for (int n = 0; n < 5; n++) atomicAdd(p + n * ttd, q[n]);
+}
+
 // Loads of the next plane's P-phase cells issued a plane ahead, per precision
 // (FP32: the per-level kernel only; it runs at two waves per SIMD with them,
 // measured 4% faster at 256^3 than at three without, while the work-list
@@ -367,6 +385,12 @@ __device__ __forceinline__ void reflux_add(double *unew, const int32_t *ghost_pa
 #ifndef RH_PAIR_WRITES
 #define RH_PAIR_WRITES 1
 #endif
+// Refluxes summed in LDS per window oct before one atomic per ghost oct and
+// variable (instead of one per fine face), for bricks whose window has at
+// most this many octs (4x4x4 bricks: 216 window octs, 8.6 KB of FP64 sums).
+#ifndef RH_REFLUX_LDS_MAX
+#define RH_REFLUX_LDS_MAX 512
+#endif
 // LDS of one brick workgroup.  NS primitive planes are kept: 2 when the slopes
 // need only face neighbours (the z neighbours of a column sit in registers),
 // 3 for the 27-point slope of slope_type 3.
@@ -387,6 +411,10 @@ template <int BX, int BY, int BZ, typename T, int NS, bool SEP> struct BrickSmem
     // (RH_PAIR_WRITES)
     static constexpr bool PW = RH_PAIR_WRITES && sizeof(T) == 8;
     double sun[PW ? 5 : 1][PW ? 4 * BX * BY : 1];
+    // coarse-fine refluxes of the brick summed per window oct (RA: windows of
+    // at most RH_REFLUX_LDS_MAX octs), flushed once per ghost oct at the end
+    static constexpr bool RA = NW <= RH_REFLUX_LDS_MAX;
+    double sacc[RA ? NW : 1][5];
 };
 
 // The FP32 per-level kernel's Courant reduction in the update phase (as the
@@ -452,6 +480,12 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     (void)CY;
     const int tid = threadIdx.x;
     for (int i = tid; i < NW; i += NT) swin[i] = a.win[(size_t)bid * NW + i];
+    // refluxes summed in LDS (atomic mode only: the deterministic mode keeps
+    // one slot per fine face)
+    constexpr bool RA = std::remove_reference_t<decltype(sm)>::RA;
+    const bool rlds = RA && GEN && a.reflux && !a.rdet;
+    if (rlds)
+        for (int i = tid; i < NW * 5; i += NT) (&sm.sacc[0][0])[i] = 0.0;
     const int ox0 = a.org[3 * bid + 0], oy0 = a.org[3 * bid + 1], oz0 = a.org[3 * bid + 2];
     // cell planes [tlo, thi] of the brick hold octs of the level; the march
     // covers them plus two halo planes on each side
@@ -834,8 +868,19 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             }
         }
         if (GEN && a.reflux) {
-            // window value and ownership of a cell of the trace region
-            auto wv = [&](int x, int y, int z) { return swin[(((z + 2) >> 1) * WY + ((y + 2) >> 1)) * WX + ((x + 2) >> 1)]; };
+            // window slot, window value and ownership of a cell of the trace region
+            auto wi = [&](int x, int y, int z) { return (((z + 2) >> 1) * WY + ((y + 2) >> 1)) * WX + ((x + 2) >> 1); };
+            auto wv = [&](int x, int y, int z) { return swin[wi(x, y, z)]; };
+            // one fine face's reflux into the ghost oct of window slot i:
+            // summed in LDS, or straight to the accumulator / parent cell
+            auto radd = [&](int i, int v, double sign, const double F[5], int slot) {
+                if (RA && rlds) {
+#pragma unroll
+                    for (int n = 0; n < 5; n++) atomicAdd(&sm.sacc[RA ? i : 0][n], sign * (F[n] * 0.125));
+                } else {
+                    reflux_add(a.unew, a.ghost_parent, a.ngridmax, v, sign, F, 8, a.racc, a.rdet, slot);
+                }
+            };
             auto owned = [&](int x, int y, int z, int v) {
                 return x >= 0 && x < 2 * BX && y >= 0 && y < 2 * BY && z >= 0 && z < 2 * BZ && v >= a.head &&
                        v < a.head + a.noct && ox0 + (x >> 1) < boxmax[0] && oy0 + (y >> 1) < boxmax[1] &&
@@ -843,34 +888,28 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
             };
             double F[5];
             if (wfx && !(myref || sref[s0][pc - 1])) {
-                const int vl = wv(cx - 1, cy, tt), vh = wv(cx, cy, tt);
+                const int il = wi(cx - 1, cy, tt), ih = wi(cx, cy, tt), vl = swin[il], vh = swin[ih];
 #pragma unroll
                 for (int n = 0; n < 5; n++) F[n] = fscale(double(fx[n]));
                 const int sub = (cy & 1) + 2 * (tt & 1);
-                if (vl > a.ngridmax && owned(cx, cy, tt, vh))
-                    reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc, a.rdet, 4 + sub);
-                if (vh > a.ngridmax && owned(cx - 1, cy, tt, vl))
-                    reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc, a.rdet, sub);
+                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) radd(il, vl, -1.0, F, 4 + sub);
+                if (vh > a.ngridmax && owned(cx - 1, cy, tt, vl)) radd(ih, vh, 1.0, F, sub);
             }
             if (wfy && !(myref || sref[s0][pc - PX])) {
-                const int vl = wv(cx, cy - 1, tt), vh = wv(cx, cy, tt);
+                const int il = wi(cx, cy - 1, tt), ih = wi(cx, cy, tt), vl = swin[il], vh = swin[ih];
 #pragma unroll
                 for (int n = 0; n < 5; n++) F[n] = fscale(double(fy[n]));
                 const int sub = (cx & 1) + 2 * (tt & 1);
-                if (vl > a.ngridmax && owned(cx, cy, tt, vh))
-                    reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc, a.rdet, 12 + sub);
-                if (vh > a.ngridmax && owned(cx, cy - 1, tt, vl))
-                    reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc, a.rdet, 8 + sub);
+                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) radd(il, vl, -1.0, F, 12 + sub);
+                if (vh > a.ngridmax && owned(cx, cy - 1, tt, vl)) radd(ih, vh, 1.0, F, 8 + sub);
             }
             if (interior && tt >= tlo && tt <= thi + 1 && !(myref || ref_prev)) {
-                const int vl = wv(cx, cy, tt - 1), vh = wv(cx, cy, tt);
+                const int il = wi(cx, cy, tt - 1), ih = wi(cx, cy, tt), vl = swin[il], vh = swin[ih];
 #pragma unroll
                 for (int n = 0; n < 5; n++) F[n] = fscale(double(fz[n]));
                 const int sub = (cx & 1) + 2 * (cy & 1);
-                if (vl > a.ngridmax && owned(cx, cy, tt, vh))
-                    reflux_add(a.unew, a.ghost_parent, a.ngridmax, vl, -1.0, F, 8, a.racc, a.rdet, 20 + sub);
-                if (vh > a.ngridmax && owned(cx, cy, tt - 1, vl))
-                    reflux_add(a.unew, a.ghost_parent, a.ngridmax, vh, 1.0, F, 8, a.racc, a.rdet, 16 + sub);
+                if (vl > a.ngridmax && owned(cx, cy, tt, vh)) radd(il, vl, -1.0, F, 20 + sub);
+                if (vh > a.ngridmax && owned(cx, cy, tt - 1, vl)) radd(ih, vh, 1.0, F, 16 + sub);
             }
         }
         if constexpr (!SEP) __syncthreads();  // the fluxes overwrite the face states in sf
@@ -962,6 +1001,22 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         }
     } else {
         for (int t = tlo; t <= thi + 2; ++t) march(t, t & 1);
+    }
+    if (RA && rlds) {
+        // one atomic per ghost oct of the window and conserved variable
+        __syncthreads();
+        for (int i = tid; i < NW; i += NT) {
+            const int v = swin[i];
+            if (v <= a.ngridmax) continue;
+            double q[5];
+            bool any = false;
+#pragma unroll
+            for (int n = 0; n < 5; n++) {
+                q[n] = sm.sacc[RA ? i : 0][n];
+                any = any || q[n] != 0.0;
+            }
+            if (any) reflux_flush(a.unew, a.ghost_parent, a.ngridmax, v, q, 8, a.racc);
+        }
     }
     if (a.courant) {
         // the brick's partials of courant_fine in a fixed reduction order
