@@ -1074,9 +1074,13 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         ga.interpol_var = s.interpol_var;
         ga.interpol_type = s.interpol_type;
         ga.smallr = smallr;
+        ga.nseg = 1;
+        ga.seg_t[0] = 0;
+        ga.seg_t[1] = lnb.nghost;
+        ga.seg_o[0] = 0;
         for (int d = 0; d < 3; d++) {
-            ga.boxmin[d] = d < s.ndim ? s.box_min[3 * (ilevel - 2) + d] : 0;
-            ga.boxmax[d] = d < s.ndim ? s.box_max[3 * (ilevel - 2) + d] : 1;
+            ga.seg_box[0][d] = d < s.ndim ? s.box_min[3 * (ilevel - 2) + d] : 0;
+            ga.seg_box[0][3 + d] = d < s.ndim ? s.box_max[3 * (ilevel - 2) + d] : 1;
             ga.btype[d][0] = s.btype[d][0];
             ga.btype[d][1] = s.btype[d][1];
         }

@@ -35,32 +35,51 @@ int efail(int code, const std::string &msg)
     return code;
 }
 
-hipError_t refresh_level_ghosts(State &s, int ilevel, double *u)
+// interpol_hydro into the ghost octs of the levels above levelmin, all of
+// them in one launch of up to kMaxGhostSegs levels: the levels read only
+// their parent levels' state, which no refresh writes.
+hipError_t refresh_ghosts(State &s, const std::vector<int> &levels, int levelmin, double *u)
 {
-    auto it = s.nbors.find(ilevel);
-    if (it == s.nbors.end() || it->second.nghost <= 0) return hipSuccess;
-    const LevelNbor &lnb = it->second;
     GhostArgs ga;
     ga.uold = u;
     ga.refined = s.refined;
-    const int gi0 = lnb.ghost_first - s.ngridmax - 1;
-    ga.src = s.ghost_src + (size_t)gi0 * 14;
-    ga.srckey = s.ghost_srckey + (size_t)gi0 * 21;
-    ga.order = s.ghost_order + gi0;
-    ga.gi0 = gi0;
-    ga.first = lnb.ghost_first;
-    ga.nghost = lnb.nghost;
+    ga.src = s.ghost_src;
+    ga.srckey = s.ghost_srckey;
+    ga.order = s.ghost_order;
+    ga.gi0 = 0;
+    ga.first = s.ngridmax + 1;
     ga.ndim = 3;
     ga.interpol_var = s.interpol_var;
     ga.interpol_type = s.interpol_type;
     ga.smallr = s.amr.smallr;
     for (int d = 0; d < 3; d++) {
-        ga.boxmin[d] = s.box_min[3 * (ilevel - 2) + d];
-        ga.boxmax[d] = s.box_max[3 * (ilevel - 2) + d];
         ga.btype[d][0] = s.btype[d][0];
         ga.btype[d][1] = s.btype[d][1];
     }
-    return launch_refresh_ghosts(ga, s.stream);
+    ga.nseg = 0;
+    ga.nghost = 0;
+    for (int l : levels) {
+        if (l <= levelmin) continue;
+        auto it = s.nbors.find(l);
+        if (it == s.nbors.end() || it->second.nghost <= 0) continue;
+        const LevelNbor &lnb = it->second;
+        if (ga.nseg == kMaxGhostSegs) {
+            hipError_t e = launch_refresh_ghosts(ga, s.stream);
+            if (e != hipSuccess) return e;
+            ga.nseg = 0;
+            ga.nghost = 0;
+        }
+        const int k = ga.nseg++;
+        ga.seg_t[k] = ga.nghost;
+        ga.seg_o[k] = lnb.ghost_first - s.ngridmax - 1;
+        for (int d = 0; d < 3; d++) {
+            ga.seg_box[k][d] = s.box_min[3 * (l - 2) + d];
+            ga.seg_box[k][3 + d] = s.box_max[3 * (l - 2) + d];
+        }
+        ga.nghost += lnb.nghost;
+        ga.seg_t[k + 1] = ga.nghost;
+    }
+    return ga.nseg > 0 ? launch_refresh_ghosts(ga, s.stream) : hipSuccess;
 }
 
 // One batched step reading the array of parity p (0: uold, 1: unew).
@@ -70,8 +89,7 @@ hipError_t enqueue_amr_step(State &s, int parity, int precision)
     double *ucur = parity ? s.unew : s.uold;
     double *unext = parity ? s.uold : s.unew;
     hipError_t e;
-    for (int l : m.levels)
-        if (l > m.levelmin && (e = refresh_level_ghosts(s, l, ucur)) != hipSuccess) return e;
+    if ((e = refresh_ghosts(s, m.levels, m.levelmin, ucur)) != hipSuccess) return e;
     for (const auto &gr : m.groups) {
         WorkList w;
         w.descs = m.d_descs + parity * m.ndesc;

@@ -145,20 +145,27 @@ hipError_t launch_courant(const double *uold, const uint8_t *refined, int head, 
 // Ghost octs of a level: fill each from its parent cell and the 2*ndim
 // neighbouring parent cells by interpol_hydro (hydro/interpol_hydro.f90:5-113).
 // src[g*7 + j] = (oct code, cell index 1-based) of u1(j); keys for boundary codes.
+// Ghost refresh of one or more levels in one launch: segment k covers
+// threads [seg_t[k], seg_t[k+1]) and the ghosts order[seg_o[k] ...] of one
+// level, whose parent level's box is seg_box[k] (min xyz, max xyz).
+constexpr int kMaxGhostSegs = 16;
 struct GhostArgs {
     double *uold;
     const uint8_t *refined;
     const int32_t *src;      // [nghost][7][2]
     const int32_t *srckey;   // [nghost][7][3]
-    const int32_t *order;    // [nghost] ghost indices (0-based, all levels) in parent-cell order
-    int gi0;                 // ghost index of the level's first ghost
-    int first;               // oct index (1-based) of the first ghost of the level
-    int nghost;
+    const int32_t *order;    // ghost indices (0-based, all levels) in parent-cell order
+    int gi0;                 // ghost index of src[0] (order values are offset by it)
+    int first;               // oct index (1-based) of ghost gi0
+    int nghost;              // threads: ghosts of all segments
     int ndim;
     int interpol_var, interpol_type;
     double smallr;
-    int boxmin[3], boxmax[3];  // coarse level box
     int btype[3][2];
+    int nseg;
+    int seg_t[kMaxGhostSegs + 1];
+    int seg_o[kMaxGhostSegs];
+    int seg_box[kMaxGhostSegs][6];
 };
 hipError_t launch_refresh_ghosts(const GhostArgs &a, hipStream_t s);
 

@@ -1546,7 +1546,11 @@ __global__ void __launch_bounds__(128) k_refresh_ghosts(const GhostArgs a)
     // so that the parent octs a wave gathers meet in one L2
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= a.nghost) return;
-    const int g = a.order[t] - a.gi0;
+    int k = 0;
+    for (int j = 1; j < a.nseg; j++) k = t >= a.seg_t[j] ? j : k;
+    const int g = a.order[a.seg_o[k] + (t - a.seg_t[k])] - a.gi0;
+    const int boxmin[3] = {a.seg_box[k][0], a.seg_box[k][1], a.seg_box[k][2]};
+    const int boxmax[3] = {a.seg_box[k][3], a.seg_box[k][4], a.seg_box[k][5]};
     constexpr int ttd = 1 << ND, nslot = 2 * ND + 1;
     const int stride = ttd * 5;
     double u1[7][5], u2[8][5];
@@ -1564,11 +1568,11 @@ __global__ void __launch_bounds__(128) k_refresh_ghosts(const GhostArgs a)
         uint8_t rf;
         if constexpr (ND == 3) {
             // select-based flip: no runtime index into u1 (which would put it in scratch)
-            const CellRef r = cell_ref3(a.uold, a.refined, code, okey, a.boxmin, a.boxmax, reflexive_mask(a.btype), c);
+            const CellRef r = cell_ref3(a.uold, a.refined, code, okey, boxmin, boxmax, reflexive_mask(a.btype), c);
             cell_load3(r, u1[j], rf);
             cell_flip(u1[j], r.flip);
         } else {
-            gather_cell(a.uold, a.refined, code, okey, ND, a.boxmin, a.boxmax, a.btype, c, u1[j], rf);
+            gather_cell(a.uold, a.refined, code, okey, ND, boxmin, boxmax, a.btype, c, u1[j], rf);
         }
     }
     interpol_hydro_dev<ND>(u1, u2, a.interpol_var, a.interpol_type, a.smallr);
