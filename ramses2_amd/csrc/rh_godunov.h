@@ -100,8 +100,10 @@ hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, 
                             hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, int *nparts = nullptr);
 hipError_t launch_godunov1d(const God1Args &a, int prec, hipStream_t s);
 // Work-list launch: items[i] = (descriptor index, brick index), one workgroup per item.
+// gen = false: every descriptor is a periodic level without refined cells or
+// refluxes (bnd, any_ref and reflux all 0), which the specialised body needs.
 hipError_t launch_godunov3d_wl(const WorkList &w, int bx, int by, int bz, int slope, int riemann, int prec,
-                               hipStream_t s);
+                               hipStream_t s, bool gen = true);
 
 // Courant reduction over cells of octs [head, head+noct): per-block partials,
 // then a fixed-order final pass -> out[0..4] = mass, ekin, eint, emag, dt.

@@ -1128,7 +1128,7 @@ __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2),
 // the FP32 HLLC kernel, one wave per SIMD less.)
 typedef const __attribute__((address_space(4))) GodArgs ConstGodArgs;
 
-template <int BX, int BY, int BZ, typename T, int RS, int NS>
+template <int BX, int BY, int BZ, typename T, int RS, int NS, bool GEN = true>
 __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), sizeof(T) == 8 ? RH_BRICK_MIN_WAVES : RH_BRICK_MIN_WAVES_F32)
     k_godunov3d_wl(const WorkList w)
 {
@@ -1138,7 +1138,7 @@ __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), sizeof(T) == 8 ? 
     if (i >= w.range[x + 1]) return;
     const int2 it = w.items[i];
     const int di = __builtin_amdgcn_readfirstlane(it.x), bi = __builtin_amdgcn_readfirstlane(it.y);
-    brick_body<BX, BY, BZ, T, RS, NS, true>(((ConstGodArgs *)w.descs)[di], bi, sm);
+    brick_body<BX, BY, BZ, T, RS, NS, true, GEN>(((ConstGodArgs *)w.descs)[di], bi, sm);
 }
 
 
@@ -1777,7 +1777,7 @@ hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, 
 }
 
 hipError_t launch_godunov3d_wl(const WorkList &w, int bx, int by, int bz, int slope, int riemann,
-                               int prec, hipStream_t s)
+                               int prec, hipStream_t s, bool gen)
 {
     if (w.range[8] <= 0) return hipSuccess;
     int maxr = 0;
@@ -1792,6 +1792,11 @@ hipError_t launch_godunov3d_wl(const WorkList &w, int bx, int by, int bz, int sl
                 hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, double, 0, 3>), g, b, 0, s, w);              \
             else                                                                                         \
                 hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, float, 0, 3>), g, b, 0, s, w);               \
+        } else if (riemann == 3 && !gen) {                                                               \
+            if (prec == 0)                                                                               \
+                hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, double, 3, 2, false>), g, b, 0, s, w);       \
+            else                                                                                         \
+                hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, float, 3, 2, false>), g, b, 0, s, w);        \
         } else if (riemann == 3) {                                                                       \
             if (prec == 0)                                                                               \
                 hipLaunchKernelGGL((k_godunov3d_wl<A, B, C, double, 3, 2>), g, b, 0, s, w);              \

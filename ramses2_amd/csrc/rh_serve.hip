@@ -329,6 +329,7 @@ int rh_batch_step(const int32_t *ids, int njobs, int precision, int nsteps)
     if (njobs == 0 || nsteps == 0) return 0;
     std::vector<State *> js(njobs);
     long nitems = 0;
+    bool gen = false;  // some job needs the general brick body
     for (int j = 0; j < njobs; j++) {
         State *s = ctx_get(ids[j]);
         if (!s || !s->job_ready)
@@ -340,6 +341,7 @@ int rh_batch_step(const int32_t *ids, int njobs, int precision, int nsteps)
             s->job.slope != js[0]->job.slope || s->job.riemann != js[0]->job.riemann)
             return efail(RH_ERR_ARG, "the jobs of one batch need the same brick shape, slope and Riemann solver");
         nitems += s->job.nbricks;
+        gen = gen || s->job.bnd || s->job.any_ref || s->job.reflux;
     }
     if (nitems > (1l << 30)) return efail(RH_ERR_ARG, "too many bricks in one batch");
     hipStream_t stream = js[0]->stream;
@@ -373,7 +375,7 @@ int rh_batch_step(const int32_t *ids, int njobs, int precision, int nsteps)
     for (int k = 0; k < nsteps; k++) {
         w.descs = b.d_descs + (k & 1) * njobs;
         RH_HIP_CHECK(launch_godunov3d_wl(w, js[0]->job_bx, js[0]->job_by, js[0]->job_bz, js[0]->job.slope,
-                                         js[0]->job.riemann, precision, stream));
+                                         js[0]->job.riemann, precision, stream, gen));
         RH_HIP_CHECK(launch_courant_reduce_batch(b.d_cjobs, njobs, stream));
     }
     if (nsteps & 1)
