@@ -269,3 +269,53 @@ def test_uniform_omp_baseline_equals_godunov_fine(threads):
     b.set_unew(4)
     b.godunov_uniform_omp(4, dt, po, (mesh.octmap + 1).astype(np.int32), threads)
     assert np.array_equal(a.unew, b.unew)
+
+
+def _linear_u1(ndim, a0, g):
+    """interpol_hydro's input for a field linear in each conserved variable:
+    cell 0 the parent cell, cells 2d-1 and 2d its neighbours below and above
+    in dimension d (one coarse cell away), value a0 + (offset) * g[d]."""
+    u1 = np.zeros((2 * ndim + 1, 5))
+    u1[0] = a0
+    for d in range(ndim):
+        u1[2 * d + 1] = a0 - g[d]
+        u1[2 * d + 2] = a0 + g[d]
+    return u1
+
+
+def _children(ndim, a0, g, scale=1.0):
+    """Children of the parent cell at offsets of a quarter coarse cell:
+    a0 + scale * sum_d (+-1/4) g[d], in the cell order 1 + ix + 2 iy + 4 iz."""
+    out = np.zeros((2 ** ndim, 5))
+    for ind in range(2 ** ndim):
+        s = [((ind >> d) & 1) - 0.5 for d in range(ndim)]
+        out[ind] = a0 + scale * sum(0.5 * s[d] * g[d] for d in range(ndim))
+    return out
+
+
+@pytest.mark.parametrize("ndim", [1, 3])
+def test_interpolation_of_linear_fields(ndim):
+    """interpol_hydro (hydro/interpol_hydro.f90:5-113) on conservative
+    variables (interpol_var=0): MinMod (1) and the unlimited central slope (3)
+    reproduce a linear field exactly in the children; MonCen (2) does too
+    when its corner limiter does not bite (the largest corner deviation,
+    sum_d |g_d|/2, within the largest neighbour deviation max_d |g_d|), and
+    scales every slope by max|g| / (sum|g|/2) when it does; type 0 injects
+    the parent value."""
+    rng = np.random.default_rng(7)
+    a0 = np.array([1.3, 0.2, -0.1, 0.05, 2.5])
+    for _ in range(5):
+        g = [rng.uniform(-0.1, 0.1, 5) for _ in range(ndim)]
+        for it in (1, 3):
+            u2, _ = O.interpol_hydro(ndim, _linear_u1(ndim, a0, g), 0, it)
+            assert np.allclose(u2, _children(ndim, a0, g), rtol=0, atol=1e-15), it
+        u2, _ = O.interpol_hydro(ndim, _linear_u1(ndim, a0, g), 0, 0)
+        assert np.array_equal(u2, np.tile(a0, (2 ** ndim, 1)))
+    # MonCen along one axis (never limited) and along the diagonal (limited)
+    g1 = [np.full(5, 0.08)] + [np.zeros(5)] * (ndim - 1)
+    u2, _ = O.interpol_hydro(ndim, _linear_u1(ndim, a0, g1), 0, 2)
+    assert np.allclose(u2, _children(ndim, a0, g1), rtol=0, atol=1e-15)
+    gd = [np.full(5, 0.08)] * ndim
+    u2, _ = O.interpol_hydro(ndim, _linear_u1(ndim, a0, gd), 0, 2)
+    lim = min(1.0, 0.08 / (0.5 * ndim * 0.08))
+    assert np.allclose(u2, _children(ndim, a0, gd, lim), rtol=0, atol=1e-15)
