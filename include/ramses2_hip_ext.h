@@ -42,6 +42,12 @@ int rh_set_difmag(double difmag);
  * rh_godunov and to the next rh_amr_prepare. */
 int rh_set_deterministic(int on);
 
+/* 1 (the default): levels that are periodic in every dimension, with no
+ * refined cell and no coarse-fine faces, run a build of the Godunov brick body
+ * without the boundary-ghost gather, refined-face zeroing and reflux code;
+ * 0: every level runs the general body (the results are the same bits). */
+int rh_set_specialized(int on);
+
 /* Oct-brick shape (octs per brick in x, y, z) used by the next rh_build_nbor
  * for 3D levels; the autotuner (ramses2_amd/autotune.py) picks it. */
 int rh_set_brick_shape(int bx, int by, int bz);

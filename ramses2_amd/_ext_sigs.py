@@ -12,6 +12,7 @@ SIGNATURES = {
     "rh_set_fused_courant": [_c],
     "rh_set_brick_shape": [_c, _c, _c],
     "rh_set_deterministic": [_c],
+    "rh_set_specialized": [_c],
     "rh_set_difmag": [_d],
     "rh_get_brick_shape": [_ip, _ip, _ip],
     "rh_timer_start": [],

@@ -308,6 +308,13 @@ int rh_set_deterministic(int on)
     return 0;
 }
 
+int rh_set_specialized(int on)
+{
+    if (on != 0 && on != 1) return efail(RH_ERR_ARG, "rh_set_specialized: 0 or 1");
+    g_specialized = on;
+    return 0;
+}
+
 int rh_get_brick_shape(int32_t *bx, int32_t *by, int32_t *bz)
 {
     if (bx) *bx = st().brick_bx;

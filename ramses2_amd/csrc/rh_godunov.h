@@ -99,6 +99,10 @@ struct God1Args {
 hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, hipStream_t s,
                             hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, int *nparts = nullptr);
 hipError_t launch_godunov1d(const God1Args &a, int prec, hipStream_t s);
+// The specialised brick bodies (GEN = false) are used where a level allows
+// them unless this is 0 (rh_set_specialized; the general body then runs
+// everywhere, for checking that both give the same bits).
+extern int g_specialized;
 // Work-list launch: items[i] = (descriptor index, brick index), one workgroup per item.
 // gen = false: every descriptor is a periodic level without refined cells or
 // refluxes (bnd, any_ref and reflux all 0), which the specialised body needs.

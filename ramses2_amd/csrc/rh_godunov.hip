@@ -1737,6 +1737,8 @@ static void launch_brick(K kernel, int nt, const GodArgs &a0, hipStream_t s, hip
     launch_ev(kernel, g, b, s, e0, e1, a);
 }
 
+int g_specialized = 1;
+
 hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, hipStream_t s, hipEvent_t e0,
                             hipEvent_t e1, int *nparts)
 {
@@ -1744,7 +1746,7 @@ hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, 
     if (a.nbricks <= 0) return hipSuccess;
     // the specialised body: periodic in every dimension, no refined cells,
     // no refluxes
-    const bool gen = a.bnd || a.any_ref || a.reflux;
+    const bool gen = a.bnd || a.any_ref || a.reflux || !g_specialized;
 #define RH_LAUNCH_SHAPE(A, B, C)                                                                         \
     if (bx == A && by == B && bz == C) {                                                                 \
         constexpr int nt = (2 * A + 2) * (2 * B + 2);                                                     \
@@ -1780,6 +1782,7 @@ hipError_t launch_godunov3d_wl(const WorkList &w, int bx, int by, int bz, int sl
                                int prec, hipStream_t s, bool gen)
 {
     if (w.range[8] <= 0) return hipSuccess;
+    gen = gen || !g_specialized;
     int maxr = 0;
     for (int r = 0; r < 8; r++) maxr = w.range[r + 1] - w.range[r] > maxr ? w.range[r + 1] - w.range[r] : maxr;
     const int nblocks = 8 * maxr;

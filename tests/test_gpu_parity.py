@@ -313,3 +313,26 @@ def test_artificial_diffusion_fp64(riemann):
     finally:
         from ramses2_amd._lib import call
         call("rh_set_difmag", 0.0)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_specialized_body_gives_the_same_bits(precision):
+    """The brick body specialised for periodic levels without refined cells or
+    refluxes (the default for such levels) against the general body
+    (rh_set_specialized(0)) on the same state: bit for bit, three steps of the
+    device-resident level stepping, FP64 and mixed."""
+    from ramses2_amd._lib import call
+    mesh = UniformMesh(3, 5)
+    cells = ic.turbulence_3d(32, seed=17)
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
+    out = []
+    for spec in (1, 0):
+        call("rh_set_specialized", spec)
+        try:
+            lev = UniformLevel(mesh, cells, p)
+            dt, t, sums, _ = lev.run_device(3, precision=precision)
+            out.append((lev.state(), dt, t))
+        finally:
+            call("rh_set_specialized", 1)
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1] and out[0][2] == out[1][2]
