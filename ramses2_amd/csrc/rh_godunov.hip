@@ -391,32 +391,12 @@ __device__ __forceinline__ void reflux_flush(double *unew, const int32_t *ghost_
 #ifndef RH_REFLUX_LDS_MAX
 #define RH_REFLUX_LDS_MAX 512
 #endif
-// LDS of one brick workgroup.  NS primitive planes are kept: 2 when the slopes
-// need only face neighbours (the z neighbours of a column sit in registers),
-// 3 for the 27-point slope of slope_type 3.
-// SEP: face fluxes get their own buffer (sff) instead of overwriting the
-// face states in sf, which saves one barrier per plane.  Measured: no gain
-// (256^3 FP64 and mixed within 1%), so off by default; the barriers are not
-// what limits the kernel.
-template <int BX, int BY, int BZ, typename T, int NS, bool SEP> struct BrickSmem {
-    static constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
-    static constexpr int PX = 2 * BX + 4, PY = 2 * BY + 4, NP = PX * PY;
-    static constexpr int WX = BX + 2, WY = BY + 2, WZ = BZ + 2, NW = WX * WY * WZ;
-    int32_t swin[NW];
-    T sq[NS][5][NP];
-    uint8_t sref[NS][NP];
-    T sf[2][5][NT];
-    T sff[SEP ? 2 : 1][5][SEP ? NT : 1];
-    // new state of the lower z-plane of the oct layer, per interior column
-    // (RH_PAIR_WRITES)
-    static constexpr bool PW = RH_PAIR_WRITES && sizeof(T) == 8;
-    double sun[PW ? 5 : 1][PW ? 4 * BX * BY : 1];
-    // coarse-fine refluxes of the brick summed per window oct (RA: windows of
-    // at most RH_REFLUX_LDS_MAX octs), flushed once per ghost oct at the end
-    static constexpr bool RA = NW <= RH_REFLUX_LDS_MAX;
-    double sacc[RA ? NW : 1][5];
-};
-
+// The update's base values (the cell's uold) taken from the P phase's loads
+// through LDS instead of a second global load at the update, where the
+// update adds to uold (device-resident steps).
+#ifndef RH_UB_LDS
+#define RH_UB_LDS 1
+#endif
 // The FP32 per-level kernel's Courant reduction in the update phase (as the
 // FP64 kernel does) instead of re-reading the brick after the march: it costs
 // the kernel a wave per SIMD, which the uniform level step recovers (step
@@ -437,6 +417,85 @@ constexpr bool f32_three_waves()
     return sizeof(T) == 4 && !WL && BX == 8 && BY == 6 && BZ == 16;
 }
 
+// Minimum waves per SIMD requested from the register allocator for the brick
+// kernels (occupancy vs. register spills; chosen by measurement).
+#ifndef RH_BRICK_MIN_WAVES_F32
+#define RH_BRICK_MIN_WAVES_F32 2
+#endif
+#ifndef RH_BRICK_MIN_WAVES
+#define RH_BRICK_MIN_WAVES 2
+#endif
+
+// Waves per SIMD a brick kernel runs at: the per-level kernels at their
+// launch bound; the FP32 work-list kernel at four (about 116-135 VGPRs).
+template <int BX, int BY, int BZ, typename T, bool WL> constexpr int brick_waves()
+{
+    return sizeof(T) == 8 ? RH_BRICK_MIN_WAVES
+                          : (WL ? 4 : (f32_three_waves<BX, BY, BZ, T, WL>() ? 3 : RH_BRICK_MIN_WAVES_F32));
+}
+
+// Halo-wave thread order: the 4 BX BY interior columns fill whole waves and
+// the halo columns fit one more (8x6 bricks, 192 + 60 threads; 4x4, 64 + 36).
+template <int BX, int BY, int RS, int NS> constexpr bool brick_hw()
+{
+    return (4 * BX * BY) % 64 == 0 && (2 * BX + 2) * (2 * BY + 2) - 4 * BX * BY <= 64 && RS == 3 && NS == 2;
+}
+
+// The next-plane prefetch: the HLLC variants of the FP64 kernels and of the
+// FP32 per-level kernel, with a thread for every cell of the P-phase ring.
+template <int BX, int BY, typename T, int RS, int NS, bool WL> constexpr bool brick_prefetch()
+{
+    constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY, NI = 4 * BX * BY;
+    constexpr bool HW = brick_hw<BX, BY, RS, NS>();
+    constexpr int NRING = HW ? 4 * (BX + BY) : 2 * (CX + CY);
+    return (sizeof(T) == 8 ? RH_PREFETCH_F64 : RH_PREFETCH_F32 && !WL) && RS == 3 && NS == 2 &&
+           (HW ? NRING <= NT - NI : NRING <= NT);
+}
+
+// LDS of one brick workgroup.  NS primitive planes are kept: 2 when the slopes
+// need only face neighbours (the z neighbours of a column sit in registers),
+// 3 for the 27-point slope of slope_type 3.
+// SEP: face fluxes get their own buffer (sff) instead of overwriting the
+// face states in sf, which saves one barrier per plane.  Measured: no gain
+// (256^3 FP64 and mixed within 1%), so off by default; the barriers are not
+// what limits the kernel.
+template <int BX, int BY, int BZ, typename T, int NS, bool SEP> struct BrickSmemCore {
+    static constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
+    static constexpr int PX = 2 * BX + 4, PY = 2 * BY + 4, NP = PX * PY;
+    static constexpr int WX = BX + 2, WY = BY + 2, WZ = BZ + 2, NW = WX * WY * WZ;
+    int32_t swin[NW];
+    T sq[NS][5][NP];
+    uint8_t sref[NS][NP];
+    T sf[2][5][NT];
+    T sff[SEP ? 2 : 1][5][SEP ? NT : 1];
+    // new state of the lower z-plane of the oct layer, per interior column
+    // (RH_PAIR_WRITES)
+    static constexpr bool PW = RH_PAIR_WRITES && sizeof(T) == 8;
+    double sun[PW ? 5 : 1][PW ? 4 * BX * BY : 1];
+};
+// The optional buffers are added only while the workgroups the kernel's
+// waves per SIMD call for (WPS) still fit the 160 KB of LDS of a CU, so that
+// they never cost occupancy:
+// - sacc (RA): the coarse-fine refluxes of the brick summed per window oct,
+//   flushed once per ghost oct at the end (windows of at most
+//   RH_REFLUX_LDS_MAX octs);
+// - sub (UBL, kernels with the next-plane prefetch, UBOK): the conserved
+//   state of each interior column's cell on the last two planes, kept from
+//   the P phase's loads as the update's base values.
+template <int BX, int BY, int BZ, typename T, int NS, bool SEP, int WPS, bool UBOK>
+struct BrickSmem : BrickSmemCore<BX, BY, BZ, T, NS, SEP> {
+    using Core = BrickSmemCore<BX, BY, BZ, T, NS, SEP>;
+    static constexpr int NW = Core::NW, NT = Core::NT;
+    static constexpr int WGS = (4 * WPS) / ((NT + 63) / 64) > 0 ? (4 * WPS) / ((NT + 63) / 64) : 1;
+    static constexpr size_t LIM = (size_t)160 * 1024 / WGS;
+    static constexpr bool RA = NW <= RH_REFLUX_LDS_MAX && sizeof(Core) + (size_t)NW * 40 <= LIM;
+    static constexpr size_t RAB = RA ? (size_t)NW * 40 : 40;
+    static constexpr bool UBL =
+        UBOK && RH_UB_LDS && NT <= 256 && sizeof(Core) + RAB + (size_t)80 * 4 * BX * BY <= LIM;
+    double sacc[RA ? NW : 1][5];
+    double sub[UBL ? 2 : 1][5][UBL ? 4 * BX * BY : 1];
+};
+
 // Separate flux buffer where the LDS allows it: the two-plane ring (the
 // 27-point slope's three planes are the big ones) within the 160 KB of a CU.
 #ifndef RH_SEP_FLUX
@@ -444,8 +503,17 @@ constexpr bool f32_three_waves()
 #endif
 template <int BX, int BY, int BZ, typename T, int NS> constexpr bool sep_flux()
 {
-    return RH_SEP_FLUX && NS == 2 && sizeof(BrickSmem<BX, BY, BZ, T, NS, true>) <= 160 * 1024;
+    return RH_SEP_FLUX && NS == 2 && sizeof(BrickSmemCore<BX, BY, BZ, T, NS, true>) <= 160 * 1024;
 }
+
+// The LDS of the per-level (WL = false) or work-list kernel.  The base
+// values through LDS (UBL) pay in the FP32 kernel, whose three waves per SIMD
+// leave no registers to load them early: at 256^3 the mixed kernel 0.666 ->
+// 0.618 ms.  The FP64 kernels load them early into registers; through LDS
+// they measured the same at 256^3 and 4% slower in the AMR work-list step.
+template <int BX, int BY, int BZ, typename T, int RS, int NS, bool WL>
+using BrickLds = BrickSmem<BX, BY, BZ, T, NS, sep_flux<BX, BY, BZ, T, NS>(), brick_waves<BX, BY, BZ, T, WL>(),
+                           brick_prefetch<BX, BY, T, RS, NS, WL>() && sizeof(T) == 4>;
 
 __device__ __forceinline__ int ring_slot2(int t) { return t & 1; }
 __device__ __forceinline__ int ring_slot3(int t) { return (t + 3) % 3; }
@@ -461,7 +529,7 @@ __device__ __forceinline__ int ring_slot3(int t) { return (t + 3) % 3; }
 // reflux code, the FP64 kernel needs 224 instead of 245 VGPRs and no SGPR
 // spills (39 with them), and runs 2-3% faster.  The launch checks the level.
 template <int BX, int BY, int BZ, typename T, int RS, int NS, bool WL, bool GEN = true, typename A>
-__device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY, BZ, T, NS, sep_flux<BX, BY, BZ, T, NS>()> &sm,
+__device__ __forceinline__ void brick_body(A &a, const int bid, BrickLds<BX, BY, BZ, T, RS, NS, WL> &sm,
                                            const int part = -1, const int pslot = -1)
 {
     constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
@@ -534,7 +602,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     // one face per column and no update, runs one solver pass instead of three
     // and skips the update; it takes the P-phase ring instead.
     constexpr int NI = 4 * BX * BY;
-    constexpr bool HW = NI % 64 == 0 && NT - NI <= 64 && RS == 3 && NS == 2;
+    constexpr bool HW = brick_hw<BX, BY, RS, NS>();
     int cx, cy;
     if constexpr (HW) {
         if (tid < NI) {
@@ -646,11 +714,15 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     // drops to two waves per SIMD with it and is faster for it; the FP32
     // work-list kernel and the other variants load at use, as they would
     // spill or lose waves (it also needs a thread for every ring cell)
-    constexpr bool PF = (sizeof(T) == 8 ? RH_PREFETCH_F64 : RH_PREFETCH_F32 && !WL) && RS == 3 && NS == 2 &&
-                        (HW ? NRING <= NT - NI : NRING <= NT);
+    constexpr bool PF = brick_prefetch<BX, BY, T, RS, NS, WL>();
     double u_o[5], u_r[5];
     uint8_t rf_o, rf_r = 0;
     int fl_o, fl_r = -1;
+    // the update's base values from the P phase (through LDS) when the update
+    // adds to uold; the own column's cell of plane t is u_o in the P phase
+    constexpr bool UBL = std::remove_reference_t<decltype(sm)>::UBL && PF;
+    const bool ubl = UBL && a.ubase == a.uold;
+    const int icol = interior ? cy * (2 * BX) + cx : 0;
     if (PF) {
         fetch(pc, tlo - 2, u_o, rf_o, fl_o);
         if (ring) fetch(ring_i, tlo - 2, u_r, rf_r, fl_r);
@@ -679,6 +751,9 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         } else {
             double q[5];
             if (bnd && fl_o >= 0) cell_flip(u_o, fl_o);  // reflexive ghosts only: skipped by waves without any
+            if (ubl && interior)
+#pragma unroll
+                for (int n = 0; n < 5; n++) sm.sub[UBL ? (t & 1) : 0][n][UBL ? icol : 0] = u_o[n];
             prim_from_cons<sizeof(T) == 4>(u_o, q, a.smallr, smalle, a.gamma, gdt);
 #pragma unroll
             for (int n = 0; n < 5; n++) {
@@ -743,7 +818,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         constexpr bool UB_EARLY = PF && !f32_three_waves<BX, BY, BZ, T, WL>();
         double ub[5];
         if constexpr (UB_EARLY) {
-            if (interior && plane_out) {
+            if (!ubl && interior && plane_out) {
                 const int ox = (cx >> 1) + 1, oy = (cy >> 1) + 1, oz = (tt >> 1) + 1;
                 const int v = swin[(oz * WY + oy) * WX + ox];
                 const int ind = (cx & 1) + 2 * (cy & 1) + 4 * (tt & 1);
@@ -924,7 +999,10 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
         if (interior) {
             double part[5] = {0, 0, 0, 0, 0};
             if (plane_out) {
-                if constexpr (!UB_EARLY) {
+                if (ubl) {
+#pragma unroll
+                    for (int n = 0; n < 5; n++) ub[n] = sm.sub[UBL ? (tt & 1) : 0][n][UBL ? icol : 0];
+                } else if constexpr (!UB_EARLY) {
                     const int ox = (cx >> 1) + 1, oy = (cy >> 1) + 1, oz = (tt >> 1) + 1;
                     const int v = swin[(oz * WY + oy) * WX + ox];
                     const int ind = (cx & 1) + 2 * (cy & 1) + 4 * (tt & 1);
@@ -1067,15 +1145,6 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickSmem<BX, BY
     }
 }
 
-// Minimum waves per SIMD requested from the register allocator for the brick
-// kernels (occupancy vs. register spills; chosen by measurement).
-#ifndef RH_BRICK_MIN_WAVES_F32
-#define RH_BRICK_MIN_WAVES_F32 2
-#endif
-#ifndef RH_BRICK_MIN_WAVES
-#define RH_BRICK_MIN_WAVES 2
-#endif
-
 template <int BX, int BY, int BZ, typename T, int RS, int NS, bool GEN = true>
 __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2),
                                   (sizeof(T) == 8                              ? RH_BRICK_MIN_WAVES
@@ -1083,7 +1152,7 @@ __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2),
                                                                              : RH_BRICK_MIN_WAVES_F32))
     k_godunov3d(const GodArgs a)
 {
-    __shared__ BrickSmem<BX, BY, BZ, T, NS, sep_flux<BX, BY, BZ, T, NS>()> sm;
+    __shared__ BrickLds<BX, BY, BZ, T, RS, NS, false> sm;
     // XCD-aware brick assignment: consecutive bricks to the same XCD so that
     // their shared halo octs meet in that XCD's L2.
     int bid = blockIdx.x;
@@ -1132,7 +1201,7 @@ template <int BX, int BY, int BZ, typename T, int RS, int NS, bool GEN = true>
 __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), sizeof(T) == 8 ? RH_BRICK_MIN_WAVES : RH_BRICK_MIN_WAVES_F32)
     k_godunov3d_wl(const WorkList w)
 {
-    __shared__ BrickSmem<BX, BY, BZ, T, NS, sep_flux<BX, BY, BZ, T, NS>()> sm;
+    __shared__ BrickLds<BX, BY, BZ, T, RS, NS, true> sm;
     const int x = blockIdx.x & 7, l = blockIdx.x >> 3;
     const int i = w.range[x] + l;
     if (i >= w.range[x + 1]) return;
