@@ -105,7 +105,8 @@ hipError_t launch_godunov1d(const God1Args &a, int prec, hipStream_t s);
 extern int g_specialized;
 // Work-list launch: items[i] = (descriptor index, brick index), one workgroup per item.
 // gen = false: every descriptor is a periodic level without refined cells or
-// refluxes (bnd, any_ref and reflux all 0), which the specialised body needs.
+// refluxes (bnd, any_ref and reflux all 0) whose update adds to uold (ubase ==
+// uold), which the specialised body needs.
 hipError_t launch_godunov3d_wl(const WorkList &w, int bx, int by, int bz, int slope, int riemann, int prec,
                                hipStream_t s, bool gen = true);
 

@@ -508,12 +508,16 @@ template <int BX, int BY, int BZ, typename T, int NS> constexpr bool sep_flux()
 
 // The LDS of the per-level (WL = false) or work-list kernel.  The base
 // values through LDS (UBL) pay in the FP32 kernel, whose three waves per SIMD
-// leave no registers to load them early: at 256^3 the mixed kernel 0.666 ->
-// 0.618 ms.  The FP64 kernels load them early into registers; through LDS
-// they measured the same at 256^3 and 4% slower in the AMR work-list step.
-template <int BX, int BY, int BZ, typename T, int RS, int NS, bool WL>
+// leave no registers to load them early (a load at the update waits, by the
+// in-order memory counter, for the next-plane prefetch issued before it): at
+// 256^3 the mixed kernel 0.666 -> 0.618 ms.  The general FP64 body loads them
+// early into registers (through LDS behind a runtime test it measured 4%
+// slower in the AMR work-list step: the compiler then waits for the early
+// loads of the other path); the specialised FP64 body (GEN = false, launched
+// only where the update adds to uold) takes them through LDS.
+template <int BX, int BY, int BZ, typename T, int RS, int NS, bool WL, bool GEN>
 using BrickLds = BrickSmem<BX, BY, BZ, T, NS, sep_flux<BX, BY, BZ, T, NS>(), brick_waves<BX, BY, BZ, T, WL>(),
-                           brick_prefetch<BX, BY, T, RS, NS, WL>() && sizeof(T) == 4>;
+                           brick_prefetch<BX, BY, T, RS, NS, WL>() && (sizeof(T) == 4 || !GEN)>;
 
 __device__ __forceinline__ int ring_slot2(int t) { return t & 1; }
 __device__ __forceinline__ int ring_slot3(int t) { return (t + 3) % 3; }
@@ -529,7 +533,7 @@ __device__ __forceinline__ int ring_slot3(int t) { return (t + 3) % 3; }
 // reflux code, the FP64 kernel needs 224 instead of 245 VGPRs and no SGPR
 // spills (39 with them), and runs 2-3% faster.  The launch checks the level.
 template <int BX, int BY, int BZ, typename T, int RS, int NS, bool WL, bool GEN = true, typename A>
-__device__ __forceinline__ void brick_body(A &a, const int bid, BrickLds<BX, BY, BZ, T, RS, NS, WL> &sm,
+__device__ __forceinline__ void brick_body(A &a, const int bid, BrickLds<BX, BY, BZ, T, RS, NS, WL, GEN> &sm,
                                            const int part = -1, const int pslot = -1)
 {
     constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
@@ -721,7 +725,7 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickLds<BX, BY,
     // the update's base values from the P phase (through LDS) when the update
     // adds to uold; the own column's cell of plane t is u_o in the P phase
     constexpr bool UBL = std::remove_reference_t<decltype(sm)>::UBL && PF;
-    const bool ubl = UBL && a.ubase == a.uold;
+    const bool ubl = UBL && (!GEN || a.ubase == a.uold);  // the specialised body is launched with ubase == uold
     const int icol = interior ? cy * (2 * BX) + cx : 0;
     if (PF) {
         fetch(pc, tlo - 2, u_o, rf_o, fl_o);
@@ -1152,7 +1156,7 @@ __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2),
                                                                              : RH_BRICK_MIN_WAVES_F32))
     k_godunov3d(const GodArgs a)
 {
-    __shared__ BrickLds<BX, BY, BZ, T, RS, NS, false> sm;
+    __shared__ BrickLds<BX, BY, BZ, T, RS, NS, false, GEN> sm;
     // XCD-aware brick assignment: consecutive bricks to the same XCD so that
     // their shared halo octs meet in that XCD's L2.
     int bid = blockIdx.x;
@@ -1201,7 +1205,7 @@ template <int BX, int BY, int BZ, typename T, int RS, int NS, bool GEN = true>
 __global__ void __launch_bounds__((2 * BX + 2) * (2 * BY + 2), sizeof(T) == 8 ? RH_BRICK_MIN_WAVES : RH_BRICK_MIN_WAVES_F32)
     k_godunov3d_wl(const WorkList w)
 {
-    __shared__ BrickLds<BX, BY, BZ, T, RS, NS, true> sm;
+    __shared__ BrickLds<BX, BY, BZ, T, RS, NS, true, GEN> sm;
     const int x = blockIdx.x & 7, l = blockIdx.x >> 3;
     const int i = w.range[x] + l;
     if (i >= w.range[x + 1]) return;
@@ -1815,7 +1819,7 @@ hipError_t launch_godunov3d(const GodArgs &a, int bx, int by, int bz, int prec, 
     if (a.nbricks <= 0) return hipSuccess;
     // the specialised body: periodic in every dimension, no refined cells,
     // no refluxes
-    const bool gen = a.bnd || a.any_ref || a.reflux || !g_specialized;
+    const bool gen = a.bnd || a.any_ref || a.reflux || a.ubase != a.uold || !g_specialized;
 #define RH_LAUNCH_SHAPE(A, B, C)                                                                         \
     if (bx == A && by == B && bz == C) {                                                                 \
         constexpr int nt = (2 * A + 2) * (2 * B + 2);                                                     \

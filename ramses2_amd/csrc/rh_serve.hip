@@ -341,7 +341,7 @@ int rh_batch_step(const int32_t *ids, int njobs, int precision, int nsteps)
             s->job.slope != js[0]->job.slope || s->job.riemann != js[0]->job.riemann)
             return efail(RH_ERR_ARG, "the jobs of one batch need the same brick shape, slope and Riemann solver");
         nitems += s->job.nbricks;
-        gen = gen || s->job.bnd || s->job.any_ref || s->job.reflux;
+        gen = gen || s->job.bnd || s->job.any_ref || s->job.reflux;  // (batches add to uold: ubase == uold)
     }
     if (nitems > (1l << 30)) return efail(RH_ERR_ARG, "too many bricks in one batch");
     hipStream_t stream = js[0]->stream;
