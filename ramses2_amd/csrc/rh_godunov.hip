@@ -381,10 +381,19 @@ __device__ __forceinline__ void reflux_flush(double *unew, const int32_t *ghost_
 #endif
 
 // unew stored one oct layer (two z-planes) at a time, in whole 64-byte lines
-// (FP64 kernels: the FP32 kernel of three waves per SIMD would spill).
+// (pair_writes).
 #ifndef RH_PAIR_WRITES
 #define RH_PAIR_WRITES 1
 #endif
+// Whole-line stores: the FP64 kernels (writes 1.30x -> 0.97x of the
+// algorithmic bytes, the same time), the FP32 per-level kernel and the FP32
+// specialised work-list kernel (serving): the mixed kernel 0.624 -> 0.581 ms
+// at 256^3, serving's mixed capacity +7%; the FP32 general work-list kernel
+// (AMR) measured 7.6% slower with them and stores each half as it comes.
+template <typename T, bool WL, bool GEN> constexpr bool pair_writes()
+{
+    return RH_PAIR_WRITES && (sizeof(T) == 8 || !WL || !GEN);
+}
 // Refluxes summed in LDS per window oct before one atomic per ghost oct and
 // variable (instead of one per fine face), for bricks whose window has at
 // most this many octs (4x4x4 bricks: 216 window octs, 8.6 KB of FP64 sums).
@@ -459,7 +468,7 @@ template <int BX, int BY, typename T, int RS, int NS, bool WL> constexpr bool br
 // face states in sf, which saves one barrier per plane.  Measured: no gain
 // (256^3 FP64 and mixed within 1%), so off by default; the barriers are not
 // what limits the kernel.
-template <int BX, int BY, int BZ, typename T, int NS, bool SEP> struct BrickSmemCore {
+template <int BX, int BY, int BZ, typename T, int NS, bool SEP, bool PWT> struct BrickSmemCore {
     static constexpr int CX = 2 * BX + 2, CY = 2 * BY + 2, NT = CX * CY;
     static constexpr int PX = 2 * BX + 4, PY = 2 * BY + 4, NP = PX * PY;
     static constexpr int WX = BX + 2, WY = BY + 2, WZ = BZ + 2, NW = WX * WY * WZ;
@@ -470,7 +479,7 @@ template <int BX, int BY, int BZ, typename T, int NS, bool SEP> struct BrickSmem
     T sff[SEP ? 2 : 1][5][SEP ? NT : 1];
     // new state of the lower z-plane of the oct layer, per interior column
     // (RH_PAIR_WRITES)
-    static constexpr bool PW = RH_PAIR_WRITES && sizeof(T) == 8;
+    static constexpr bool PW = PWT;
     double sun[PW ? 5 : 1][PW ? 4 * BX * BY : 1];
 };
 // The optional buffers are added only while the workgroups the kernel's
@@ -482,9 +491,9 @@ template <int BX, int BY, int BZ, typename T, int NS, bool SEP> struct BrickSmem
 // - sub (UBL, kernels with the next-plane prefetch, UBOK): the conserved
 //   state of each interior column's cell on the last two planes, kept from
 //   the P phase's loads as the update's base values.
-template <int BX, int BY, int BZ, typename T, int NS, bool SEP, int WPS, bool UBOK>
-struct BrickSmem : BrickSmemCore<BX, BY, BZ, T, NS, SEP> {
-    using Core = BrickSmemCore<BX, BY, BZ, T, NS, SEP>;
+template <int BX, int BY, int BZ, typename T, int NS, bool SEP, int WPS, bool UBOK, bool PWT>
+struct BrickSmem : BrickSmemCore<BX, BY, BZ, T, NS, SEP, PWT> {
+    using Core = BrickSmemCore<BX, BY, BZ, T, NS, SEP, PWT>;
     static constexpr int NW = Core::NW, NT = Core::NT;
     static constexpr int WGS = (4 * WPS) / ((NT + 63) / 64) > 0 ? (4 * WPS) / ((NT + 63) / 64) : 1;
     static constexpr size_t LIM = (size_t)160 * 1024 / WGS;
@@ -503,7 +512,7 @@ struct BrickSmem : BrickSmemCore<BX, BY, BZ, T, NS, SEP> {
 #endif
 template <int BX, int BY, int BZ, typename T, int NS> constexpr bool sep_flux()
 {
-    return RH_SEP_FLUX && NS == 2 && sizeof(BrickSmemCore<BX, BY, BZ, T, NS, true>) <= 160 * 1024;
+    return RH_SEP_FLUX && NS == 2 && sizeof(BrickSmemCore<BX, BY, BZ, T, NS, true, false>) <= 160 * 1024;
 }
 
 // The LDS of the per-level (WL = false) or work-list kernel.  The base
@@ -517,7 +526,8 @@ template <int BX, int BY, int BZ, typename T, int NS> constexpr bool sep_flux()
 // only where the update adds to uold) takes them through LDS.
 template <int BX, int BY, int BZ, typename T, int RS, int NS, bool WL, bool GEN>
 using BrickLds = BrickSmem<BX, BY, BZ, T, NS, sep_flux<BX, BY, BZ, T, NS>(), brick_waves<BX, BY, BZ, T, WL>(),
-                           brick_prefetch<BX, BY, T, RS, NS, WL>() && (sizeof(T) == 4 || !GEN)>;
+                           brick_prefetch<BX, BY, T, RS, NS, WL>() && (sizeof(T) == 4 || !GEN),
+                           pair_writes<T, WL, GEN>()>;
 
 __device__ __forceinline__ int ring_slot2(int t) { return t & 1; }
 __device__ __forceinline__ int ring_slot3(int t) { return (t + 3) % 3; }
