@@ -460,8 +460,8 @@ def run_serving(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (each GPU box has one MI355X); "
-              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        print("bench.py: the hydro path runs on one GPU (one MI355X); multi-GPU sharding is a design "
+              "only, so --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     from ramses2_amd import HydroParams, UniformMesh, ic
 

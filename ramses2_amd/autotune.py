@@ -25,7 +25,7 @@ DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", 
 
 
 def compiled_shapes():
-    """The shapes of SHAPES that this build of the library has kernels for."""
+    """The shapes of SHAPES that the loaded library has kernels for."""
     out = []
     for s in SHAPES:
         v = [ctypes.c_int32() for _ in range(3)]

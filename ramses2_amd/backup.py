@@ -15,7 +15,7 @@ The byte layout restates the reference's writers and readers:
   parameters as formatted text; the reader takes time, boxlen, gamma,
   levelmin, levelmax from it.
 File numbers are title(n): five digits, zero padded (amr/title.f90).
-One file per rank; this build writes and reads one (ncpu = 1).
+One file per rank; this module writes and reads one (ncpu = 1).
 """
 import os
 import struct
