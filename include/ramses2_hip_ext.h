@@ -183,6 +183,18 @@ int rh_batch_step(const int32_t *ids, int njobs, int precision, int nsteps);
  * dt} (the last four from the last Courant reduction). */
 int rh_job_status(int id, double out[7]);
 
+/* A fence on the library stream, for a request loop that keeps one batched
+ * step in flight while it retires the jobs of the one before: records an
+ * event after every piece of work enqueued so far and, in stream order, copies
+ * the rh_job_status values of the n contexts ids[] (n may be 0) into pinned
+ * host memory the library holds.  Asynchronous; *fence names it until
+ * rh_fence_wait.  At most 64 fences are outstanding (RH_ERR_STATE beyond). */
+int rh_fence_record(const int32_t *ids, int n, int32_t *fence);
+
+/* Block until the fence has passed, write its n statuses (7 values each, in
+ * the order of ids[]) to out (may be null), and release the fence. */
+int rh_fence_wait(int32_t fence, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,8 @@ SIGNATURES = {
     "rh_job_prepare": [_c, _c, _c, _d, _d, _d, _d, _d, _dp, _c, _c],
     "rh_batch_step": [_ip, _c, _c, _c],
     "rh_job_status": [_c, _dp],
+    "rh_fence_record": [_ip, _c, _ip],
+    "rh_fence_wait": [_c, _dp],
     "rh_get_level_table": [_c, _c, _ip, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)],
     "rh_brick_lattice": [_ip, _c, _ip, _ip, _c, _c, _c, _ip, _dp],
 }

@@ -112,6 +112,18 @@ int batch_reserve(int njobs, int nitems, hipStream_t s)
     return 0;
 }
 
+// Fences of a pipelined request loop: an event on the library stream and the
+// statuses of some jobs at that point, copied by the stream into pinned host
+// memory (8 doubles per job: the 7 of rh_job_status and one spare).
+struct Fence {
+    hipEvent_t ev = nullptr;
+    double *h = nullptr;
+    int cap = 0, n = 0;
+    bool busy = false;
+};
+constexpr int kMaxFences = 64;
+std::vector<Fence> g_fences;
+
 }  // namespace
 
 extern "C" {
@@ -394,6 +406,64 @@ int rh_job_status(int id, double out[7])
     RH_HIP_CHECK(hipMemcpyAsync(v + 2, s->red_out, 5 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     RH_HIP_CHECK(hipStreamSynchronize(s->stream));
     for (int i = 0; i < 7; i++) out[i] = v[i];
+    return 0;
+}
+
+int rh_fence_record(const int32_t *ids, int n, int32_t *fence)
+{
+    if (n < 0 || (n > 0 && !ids) || !fence) return efail(RH_ERR_ARG, "rh_fence_record: bad arguments");
+    std::vector<State *> js(n);
+    for (int j = 0; j < n; j++) {
+        js[j] = ctx_get(ids[j]);
+        if (!js[j] || !js[j]->inited)
+            return efail(RH_ERR_ARG, "no initialised context with id " + std::to_string(ids[j]));
+    }
+    hipStream_t stream;
+    if (n > 0) {
+        stream = js[0]->stream;  // every context enqueues on the one library stream
+    } else {
+        if (!st().inited) return efail(RH_ERR_STATE, "rh_init has not been called");
+        stream = st().stream;
+    }
+    int f = 0;
+    while (f < (int)g_fences.size() && g_fences[f].busy) f++;
+    if (f == kMaxFences) return efail(RH_ERR_STATE, "more than 64 fences outstanding");
+    if (f == (int)g_fences.size()) g_fences.emplace_back();
+    Fence &fc = g_fences[f];
+    if (!fc.ev) RH_HIP_CHECK(hipEventCreateWithFlags(&fc.ev, hipEventDisableTiming));
+    if (n > fc.cap) {
+        // a free fence: no copy into its buffer is pending
+        const int cap = std::max(n, std::max(16, 2 * fc.cap));
+        (void)hipHostFree(fc.h);
+        fc.h = nullptr;
+        fc.cap = 0;
+        RH_HIP_CHECK(hipHostMalloc(&fc.h, (size_t)cap * 8 * sizeof(double), hipHostMallocDefault));
+        fc.cap = cap;
+    }
+    for (int j = 0; j < n; j++) {
+        const State &s = *js[j];
+        double *v = fc.h + 8 * j;
+        RH_HIP_CHECK(hipMemcpyAsync(v, s.t_dev, sizeof(double), hipMemcpyDeviceToHost, stream));
+        RH_HIP_CHECK(hipMemcpyAsync(v + 1, s.dt_dev, sizeof(double), hipMemcpyDeviceToHost, stream));
+        RH_HIP_CHECK(hipMemcpyAsync(v + 2, s.red_out, 5 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+    RH_HIP_CHECK(hipEventRecord(fc.ev, stream));
+    fc.n = n;
+    fc.busy = true;
+    *fence = f;
+    return 0;
+}
+
+int rh_fence_wait(int32_t fence, double *out)
+{
+    if (fence < 0 || fence >= (int)g_fences.size() || !g_fences[fence].busy)
+        return efail(RH_ERR_ARG, "no outstanding fence " + std::to_string(fence));
+    Fence &fc = g_fences[fence];
+    fc.busy = false;
+    RH_HIP_CHECK(hipEventSynchronize(fc.ev));
+    if (out)
+        for (int j = 0; j < fc.n; j++)
+            for (int i = 0; i < 7; i++) out[7 * j + i] = fc.h[8 * j + i];
     return 0;
 }
 

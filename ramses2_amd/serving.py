@@ -8,15 +8,17 @@ completion, in its own library context (ramses2_hip_ext.h, rh_ctx_*).
 
 Continuous batching: every tick advances all admitted jobs by one time step
 with shared launches (rh_batch_step: one work-list Godunov kernel over the
-bricks of every job, one per-job Courant reduction).  Between ticks, finished
-jobs leave and queued jobs are admitted in arrival order while a slot is free
-and their device footprint fits the HBM budget (rh_estimate_uniform_bytes).
-Work of a job does not depend on which other jobs share its launches, so a
-job's result is the same alone and in the mix.
+bricks of every job, one per-job Courant reduction).  The host keeps one
+tick in flight: while the device runs tick k, finished jobs of tick k-1 leave
+(their statuses copied by the stream at a fence, rh_fence_record) and queued
+jobs are admitted in arrival order while a slot is free and their device
+footprint fits the HBM budget (rh_estimate_uniform_bytes).  Work of a job does
+not depend on which other jobs share its launches, so a job's result is the
+same alone and in the mix.
 
-The loop is written against a small engine interface (open / step / status /
-close) so the scheduling can be tested without a GPU; HipEngine is the device
-engine.
+The loop is written against a small engine interface (open / step / fence /
+wait / close) so the scheduling can be tested without a GPU; HipEngine is the
+device engine.
 """
 import ctypes
 import time
@@ -232,10 +234,29 @@ class HipEngine:
     def sync(self):
         call("rh_device_sync")
 
+    @staticmethod
+    def _status(out):
+        return {"t": out[0], "dt_next": out[1], "mass": out[2], "ekin": out[3], "eint": out[4]}
+
     def status(self, h):
         out = np.zeros(7)
         call("rh_job_status", h, ptr(out))
-        return {"t": out[0], "dt_next": out[1], "mass": out[2], "ekin": out[3], "eint": out[4]}
+        return self._status(out)
+
+    def fence(self, handles):
+        """A fence after the work enqueued so far, holding the statuses of the
+        jobs handles at that point (rh_fence_record); returns (fence, n)."""
+        ids = np.ascontiguousarray(handles, dtype=np.int32)
+        f = ctypes.c_int32()
+        call("rh_fence_record", ptr(ids, _lib._ip), len(ids), ctypes.byref(f))
+        return f.value, len(ids)
+
+    def wait(self, fence):
+        """Block until the fence has passed; the statuses it holds."""
+        f, n = fence
+        out = np.zeros((max(n, 1), 7))
+        call("rh_fence_wait", f, ptr(out))
+        return [self._status(out[j]) for j in range(n)]
 
     def state(self, h, spec: JobSpec):
         """Final conservative cells (5, N, N, N) of the job."""
@@ -297,25 +318,54 @@ class EnsembleServer:
 
     def run(self, specs, realtime=True):
         """Serve the stream; returns {job_id: JobResult}.  realtime=False admits
-        every job at t=0 regardless of its arrival time (closed batch)."""
+        every job at t=0 regardless of its arrival time (closed batch).
+
+        One tick stays in flight: the loop enqueues tick k (the jobs' step
+        counts are known on the host, so the jobs of tick k are too), then
+        waits for tick k-1 and retires the jobs that finished with it, then
+        admits new jobs, all while the device runs tick k.  A job admitted
+        during tick k joins tick k+1."""
         eng = self.engine
         pending = deque(sorted(specs, key=lambda s: (s.arrival, s.job_id)))
         queue = deque()
-        active = []           # [spec, handle, steps_done, admitted_at, bytes]
+        active = []           # [spec, handle, ticks_enqueued, admitted_at, bytes]
+        resident = 0          # jobs holding device memory (active + finished, not yet retired)
         results = {}
         in_use = 0
+        inflight = None       # (fence, jobs that finish with the tick in flight)
         t0 = self.clock()
         ticks = 0
-        while pending or queue or active:
+        while pending or queue or active or inflight:
             now = self.clock() - t0
             while pending and (not realtime or pending[0].arrival <= now):
                 queue.append(pending.popleft())
-            # one tick of every admitted job, asynchronous on the device
-            stepped = bool(active)
-            if stepped:
+            # enqueue the next tick of every admitted job behind the one in flight
+            fence = None
+            if active:
                 eng.step([a[1] for a in active])
-            # admissions overlap the tick on the device
-            while queue and len(active) < self.max_slots:
+                ticks += 1
+                done, still = [], []
+                for a in active:
+                    a[2] += 1
+                    (done if a[2] >= a[0].nsteps else still).append(a)
+                active = still
+                fence = (eng.fence([a[1] for a in done]), done)
+            # the tick before: wait for it and retire the jobs it finished
+            if inflight:
+                f, done = inflight
+                sts = eng.wait(f)
+                for a, st in zip(done, sts):
+                    spec = a[0]
+                    state = eng.state(a[1], spec) if self.keep_states else None
+                    eng.close(a[1])
+                    in_use -= a[4]
+                    resident -= 1
+                    results[spec.job_id] = JobResult(spec.job_id, spec.ncell, spec.arrival if realtime else 0.0,
+                                                     a[3], self.clock() - t0, st, state)
+                self.trace.append((self.clock() - t0, resident, in_use))
+            inflight = fence
+            # admissions overlap the tick just enqueued
+            while queue and resident < self.max_slots:
                 need = eng.footprint(queue[0])
                 reserved = eng.template_bytes()  # the per-size template meshes, resident beside the jobs
                 if reserved + need > self.hbm_budget:
@@ -325,30 +375,11 @@ class EnsembleServer:
                     break
                 spec = queue.popleft()
                 h = eng.open(spec)
-                # a job admitted while a tick runs is not part of that tick
-                active.append([spec, h, -1 if stepped else 0, self.clock() - t0, need])
+                active.append([spec, h, 0, self.clock() - t0, need])
                 in_use += need
-            if not stepped:
-                if not active and pending:
-                    self.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
-                continue
-            eng.sync()
-            ticks += 1
-            still = []
-            for a in active:
-                a[2] += 1
-                spec = a[0]
-                if a[2] >= spec.nsteps:
-                    st = eng.status(a[1])
-                    state = eng.state(a[1], spec) if self.keep_states else None
-                    eng.close(a[1])
-                    in_use -= a[4]
-                    results[spec.job_id] = JobResult(spec.job_id, spec.ncell, spec.arrival if realtime else 0.0,
-                                                     a[3], self.clock() - t0, st, state)
-                else:
-                    still.append(a)
-            active = still
-            self.trace.append((self.clock() - t0, len(active), in_use))
+                resident += 1
+            if not active and not inflight and pending:
+                self.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
         self.ticks = ticks
         self.elapsed = self.clock() - t0
         return results

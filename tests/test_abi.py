@@ -53,6 +53,11 @@ def test_host_side_errors_without_gpu():
     bt = (ctypes.c_int32 * 6)(2, 2, 9, 2, 2, 2)
     assert L.rh_set_boundaries(bt) == 2
     assert L.rh_set_brick_shape(7, 7, 8) == 0
+    # a fence never recorded, and one of negative length
+    assert L.rh_fence_wait(3, None) == 2
+    assert b"no outstanding fence" in L.rh_last_error()
+    f = ctypes.c_int32()
+    assert L.rh_fence_record(None, -1, ctypes.byref(f)) == 2
     with pytest.raises(_lib.HipError):
         _lib.call("rh_set_precision", 3)
 

@@ -87,3 +87,38 @@ def test_footprint_estimate_bounds_allocation():
         finally:
             eng.close(h)
         assert 0.9 * est <= held <= est, (n, held, est)
+
+
+def test_fence_holds_the_statuses_of_its_point_in_the_stream():
+    """A fence recorded between two ticks holds the statuses after the first,
+    bit for bit what rh_job_status reads after a synchronisation there; a
+    fence of no jobs only orders."""
+    specs = make_stream(3, rate=0.0, seed=7, nmin=16, nmax=24, nsteps=2)
+    eng = HipEngine(P)
+    hs = [eng.open(s) for s in specs]
+    try:
+        eng.step(hs)
+        eng.sync()
+        after1 = [eng.status(h) for h in hs]
+        eng.step(hs)
+        eng.sync()
+        after2 = [eng.status(h) for h in hs]
+    finally:
+        for h in hs:
+            eng.close(h)
+    hs = [eng.open(s) for s in specs]
+    try:
+        eng.step(hs)
+        f1 = eng.fence(hs[::-1])
+        eng.step(hs)
+        f2 = eng.fence([])
+        f3 = eng.fence(hs[1:])
+        got1 = eng.wait(f1)
+        assert eng.wait(f2) == []
+        got3 = eng.wait(f3)
+    finally:
+        for h in hs:
+            eng.close(h)
+    assert got1 == after1[::-1]
+    assert got3 == after2[1:]
+    assert all(a["t"] > b["t"] > 0 for a, b in zip(after2, after1))

@@ -65,6 +65,12 @@ class CountingEngine:
         self.order = []
         self.next = 1
         self.templates = 0
+        self.next_fence = 0
+        self.outstanding = {}
+        self.max_outstanding = 0
+        self.fenced = set()
+        self.retirable = set()
+        self.log = []
 
     def footprint(self, spec):
         return 1000 * spec.ncell ** 3
@@ -79,6 +85,7 @@ class CountingEngine:
         self.steps[h] = 0
         self.bytes[h] = self.footprint(spec)
         self.order.append(spec.job_id)
+        self.log.append(("open", h))
         self.max_open = max(self.max_open, len(self.open_set))
         self.max_bytes = max(self.max_bytes, sum(self.bytes[x] for x in self.open_set))
         return h
@@ -86,9 +93,27 @@ class CountingEngine:
     def step(self, handles, nsteps=1):
         assert len(set(handles)) == len(handles)
         for h in handles:
-            assert h in self.open_set
+            assert h in self.open_set and h not in self.fenced
             self.steps[h] += nsteps
         self.clock.t += 0.001 * len(handles)
+        self.log.append(("step", len(handles)))
+
+    def fence(self, handles):
+        """The statuses at this point of the stream; a job fenced here takes
+        no further step."""
+        f = self.next_fence
+        self.next_fence += 1
+        self.outstanding[f] = (list(handles), [{"steps": self.steps[h]} for h in handles])
+        self.fenced.update(handles)
+        self.max_outstanding = max(self.max_outstanding, len(self.outstanding))
+        self.log.append(("fence", f))
+        return f
+
+    def wait(self, f):
+        self.log.append(("wait", f))
+        handles, sts = self.outstanding.pop(f)
+        self.retirable.update(handles)
+        return sts
 
     def sync(self):
         pass
@@ -100,7 +125,9 @@ class CountingEngine:
         return None
 
     def close(self, h):
+        assert h in self.retirable      # its fence has passed
         self.open_set.remove(h)
+        self.log.append(("close", h))
 
 
 class FakeClock:
@@ -141,6 +168,24 @@ def test_closed_batch_admits_everything_at_once():
     res = srv.run(specs, realtime=False)
     assert eng.max_open == 10 and srv.ticks == 3
     assert all(r.status["steps"] == 3 for r in res.values())
+    assert not eng.outstanding and eng.max_outstanding == 2
+
+
+def test_one_tick_stays_in_flight():
+    """Tick k is enqueued before the loop waits for tick k-1, and a job is
+    closed only after the fence of its last tick has passed."""
+    specs = make_stream(2, rate=1.0, seed=8, nmin=8, nmax=16, nsteps=3)
+    specs[1].nsteps = 2
+    clock = FakeClock()
+    eng = CountingEngine(clock)
+    srv = EnsembleServer(eng, max_slots=64, hbm_budget=1 << 40, clock=clock, sleep=clock.sleep)
+    res = srv.run(specs, realtime=False)
+    assert eng.log == [("open", 1), ("open", 2),
+                       ("step", 2), ("fence", 0),
+                       ("step", 2), ("fence", 1), ("wait", 0),
+                       ("step", 1), ("fence", 2), ("wait", 1), ("close", 2),
+                       ("wait", 2), ("close", 1)]
+    assert res[specs[0].job_id].status["steps"] == 3 and res[specs[1].job_id].status["steps"] == 2
 
 
 def test_admission_counts_the_resident_templates():
