@@ -236,6 +236,8 @@ __global__ void k_make_ghosts(const int32_t *__restrict__ ckey, const int32_t *_
 // neighbouring parent cells (get_twondim_nbor_parent_cell, :149-217) of each
 // new ghost: the stencil interpol_hydro reads.  counter[2] flags a missing
 // parent ("GODUNOV: parent_cell should exist", hydro/godunov_fine.f90:491).
+// gsrc receives the level's new ghosts only ([n][7][2], ghost first_gi first);
+// k_order_sources rearranges it for the interpolation.
 __global__ void k_ghost_sources(const int32_t *__restrict__ gkey, int first_gi, int n, ResolveCtx fine,
                                 ResolveCtx coarse, int32_t *gparent, int32_t *gsrc, int32_t *gsrckey, int *counter)
 {
@@ -265,8 +267,8 @@ __global__ void k_ghost_sources(const int32_t *__restrict__ gkey, int first_gi, 
             ind += (1 << d) * (kn[d] & 1);
         }
         const int code = resolve_oct(coarse, kf[0], kf[1], kf[2]);
-        gsrc[(gi * 7 + slot) * 2] = code;
-        gsrc[(gi * 7 + slot) * 2 + 1] = ind;
+        gsrc[(j * 7 + slot) * 2] = code;
+        gsrc[(j * 7 + slot) * 2 + 1] = ind;
         gsrckey[(gi * 7 + slot) * 3] = kf[0];
         gsrckey[(gi * 7 + slot) * 3 + 1] = kf[1];
         gsrckey[(gi * 7 + slot) * 3 + 2] = kf[2];
@@ -280,6 +282,20 @@ __global__ void k_ghost_sources(const int32_t *__restrict__ gkey, int first_gi, 
             atomicOr(&counter[2], 2);
         }
     }
+}
+
+// The ghost sources of one level rearranged for the interpolation, which walks
+// the ghosts in parent-cell order: word e (slot e/2, its oct code or its cell)
+// of the ghost at position base+i of that order goes to src[e*ld + base+i], so
+// that the lanes of a wave, which take consecutive positions, read
+// consecutive words.
+__global__ void k_order_sources(const int32_t *__restrict__ tmp, const int32_t *__restrict__ order, int base, int n,
+                                int ld, int32_t *__restrict__ src)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int g = order[base + i] - base;
+    for (int e = 0; e < 14; e++) src[(size_t)e * ld + base + i] = tmp[(size_t)g * 14 + e];
 }
 
 // father(oct) = parent cell of each oct of the level (update_father_array,
@@ -810,9 +826,11 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
                 }
                 const int nnew = hc[0];
                 if (nnew > 0) {
+                    int32_t *tmp = nullptr;  // the new ghosts' sources in ghost order, until rearranged
+                    RH_HIP_CHECK(dmalloc((void **)&tmp, (size_t)nnew * 14 * sizeof(int32_t)));
                     RH_HIP_CHECK(hipMemsetAsync(s.d_counter, 0, 4 * sizeof(int), s.stream));
                     hipLaunchKernelGGL(k_ghost_sources, dim3((nnew + 127) / 128), dim3(128), 0, s.stream, s.ghost_key,
-                                       base, nnew, c, cc, s.ghost_parent, s.ghost_src, s.ghost_srckey, s.d_counter);
+                                       base, nnew, c, cc, s.ghost_parent, tmp, s.ghost_srckey, s.d_counter);
                     RH_HIP_CHECK(hipGetLastError());
                     RH_HIP_CHECK(hipMemcpyAsync(hc, s.d_counter, 4 * sizeof(int), hipMemcpyDeviceToHost, s.stream));
                     RH_HIP_CHECK(hipStreamSynchronize(s.stream));
@@ -836,7 +854,11 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
                         for (int g = 0; g < nnew; g++) ord[g] += base;
                         RH_HIP_CHECK(hipMemcpyAsync(s.ghost_order + base, ord.data(), ord.size() * sizeof(int32_t),
                                                     hipMemcpyHostToDevice, s.stream));
+                        hipLaunchKernelGGL(k_order_sources, dim3((nnew + 127) / 128), dim3(128), 0, s.stream, tmp,
+                                           s.ghost_order, base, nnew, ghost_ld(s), s.ghost_src);
+                        RH_HIP_CHECK(hipGetLastError());
                         RH_HIP_CHECK(hipStreamSynchronize(s.stream));
+                        dfree(tmp);
                     }
                     for (int g = 0; g < nnew; g++) s.h_lev[s.ngridmax + base + g] = ilevel;
                     if (hc[2] & 1) {
@@ -1064,7 +1086,8 @@ int rh_godunov(int head_idx, int num_subgrids, int ngridmax, int ilevel, int lev
         ga.uold = s.uold;
         ga.refined = s.refined;
         const int gi0 = lnb.ghost_first - s.ngridmax - 1;
-        ga.src = s.ghost_src + (size_t)gi0 * 14;
+        ga.src = s.ghost_src + gi0;
+        ga.src_ld = ghost_ld(s);
         ga.srckey = s.ghost_srckey + (size_t)gi0 * 21;
         ga.order = s.ghost_order + gi0;
         ga.gi0 = gi0;

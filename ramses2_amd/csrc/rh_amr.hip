@@ -44,6 +44,7 @@ hipError_t refresh_ghosts(State &s, const std::vector<int> &levels, int levelmin
     ga.uold = u;
     ga.refined = s.refined;
     ga.src = s.ghost_src;
+    ga.src_ld = ghost_ld(s);
     ga.srckey = s.ghost_srckey;
     ga.order = s.ghost_order;
     ga.gi0 = 0;

@@ -159,7 +159,8 @@ constexpr int kMaxGhostSegs = 16;
 struct GhostArgs {
     double *uold;
     const uint8_t *refined;
-    const int32_t *src;      // [nghost][7][2]
+    const int32_t *src;      // [14][src_ld]: word e (slot e/2, oct code or cell) of each ghost, by order position
+    int src_ld;
     const int32_t *srckey;   // [nghost][7][3]
     const int32_t *order;    // ghost indices (0-based, all levels) in parent-cell order
     int gi0;                 // ghost index of src[0] (order values are offset by it)

@@ -958,9 +958,8 @@ __device__ __forceinline__ void brick_body(A &a, const int bid, BrickLds<BX, BY,
             }
         }
         if (GEN && a.reflux) {
-            // window slot, window value and ownership of a cell of the trace region
+            // window slot of a cell of the trace region
             auto wi = [&](int x, int y, int z) { return (((z + 2) >> 1) * WY + ((y + 2) >> 1)) * WX + ((x + 2) >> 1); };
-            auto wv = [&](int x, int y, int z) { return swin[wi(x, y, z)]; };
             // one fine face's reflux into the ghost oct of window slot i:
             // summed in LDS, or straight to the accumulator / parent cell
             auto radd = [&](int i, int v, double sign, const double F[5], int slot) {
@@ -1624,56 +1623,95 @@ __device__ __forceinline__ void interpol_hydro_dev(double u1[7][5], double u2[8]
 }
 
 template <int ND>
-__global__ void __launch_bounds__(128) k_refresh_ghosts(const GhostArgs a)
+__global__ void __launch_bounds__(64) k_refresh_ghosts(const GhostArgs a)
 {
     // ghosts in parent-cell order, a contiguous range of that order per XCD,
     // so that the parent octs a wave gathers meet in one L2
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= a.nghost) return;
-    int k = 0;
-    for (int j = 1; j < a.nseg; j++) k = t >= a.seg_t[j] ? j : k;
-    const int g = a.order[a.seg_o[k] + (t - a.seg_t[k])] - a.gi0;
-    const int boxmin[3] = {a.seg_box[k][0], a.seg_box[k][1], a.seg_box[k][2]};
-    const int boxmax[3] = {a.seg_box[k][3], a.seg_box[k][4], a.seg_box[k][5]};
+    const bool valid = t < a.nghost;
+    if constexpr (ND == 1) {
+        if (!valid) return;
+    }
     constexpr int ttd = 1 << ND, nslot = 2 * ND + 1;
     const int stride = ttd * 5;
-    double u1[7][5], u2[8][5];
+    double u2[8][5];
+    int g = -1;
+    if (valid) {
+        int k = 0;
+        for (int j = 1; j < a.nseg; j++) k = t >= a.seg_t[j] ? j : k;
+        const int pos = a.seg_o[k] + (t - a.seg_t[k]);
+        g = a.order[pos] - a.gi0;
+        const int boxmin[3] = {a.seg_box[k][0], a.seg_box[k][1], a.seg_box[k][2]};
+        const int boxmax[3] = {a.seg_box[k][3], a.seg_box[k][4], a.seg_box[k][5]};
+        double u1[7][5];
 #pragma unroll
-    for (int j = 0; j < nslot; j++) {
-        const int code = a.src[(g * 7 + j) * 2], ind = a.src[(g * 7 + j) * 2 + 1];
-        // the source's oct key locates boundary ghosts only (code < 0)
-        int okey[3] = {0, 0, 0};
-        if (code < 0) {
-            okey[0] = a.srckey[(g * 7 + j) * 3];
-            okey[1] = a.srckey[(g * 7 + j) * 3 + 1];
-            okey[2] = a.srckey[(g * 7 + j) * 3 + 2];
+        for (int j = 0; j < nslot; j++) {
+            const int code = a.src[(size_t)(2 * j) * a.src_ld + pos], ind = a.src[(size_t)(2 * j + 1) * a.src_ld + pos];
+            // the source's oct key locates boundary ghosts only (code < 0)
+            int okey[3] = {0, 0, 0};
+            if (code < 0) {
+                okey[0] = a.srckey[(g * 7 + j) * 3];
+                okey[1] = a.srckey[(g * 7 + j) * 3 + 1];
+                okey[2] = a.srckey[(g * 7 + j) * 3 + 2];
+            }
+            int c[3] = {(ind - 1) & 1, ((ind - 1) >> 1) & 1, ((ind - 1) >> 2) & 1};
+            uint8_t rf;
+            if constexpr (ND == 3) {
+                // select-based flip: no runtime index into u1 (which would put it in scratch)
+                const CellRef r =
+                    cell_ref3(a.uold, a.refined, code, okey, boxmin, boxmax, reflexive_mask(a.btype), c);
+                cell_load3(r, u1[j], rf);
+                cell_flip(u1[j], r.flip);
+            } else {
+                gather_cell(a.uold, a.refined, code, okey, ND, boxmin, boxmax, a.btype, c, u1[j], rf);
+            }
         }
-        int c[3] = {(ind - 1) & 1, ((ind - 1) >> 1) & 1, ((ind - 1) >> 2) & 1};
-        uint8_t rf;
-        if constexpr (ND == 3) {
-            // select-based flip: no runtime index into u1 (which would put it in scratch)
-            const CellRef r = cell_ref3(a.uold, a.refined, code, okey, boxmin, boxmax, reflexive_mask(a.btype), c);
-            cell_load3(r, u1[j], rf);
-            cell_flip(u1[j], r.flip);
-        } else {
-            gather_cell(a.uold, a.refined, code, okey, ND, boxmin, boxmax, a.btype, c, u1[j], rf);
-        }
+        interpol_hydro_dev<ND>(u1, u2, a.interpol_var, a.interpol_type, a.smallr);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+#pragma unroll
+            for (int n = 0; n < 5; n++) u2[c][n] = 0.0;
     }
-    interpol_hydro_dev<ND>(u1, u2, a.interpol_var, a.interpol_type, a.smallr);
-    double *p = a.uold + (size_t)(a.first - 1 + g) * stride;
+    if constexpr (ND == 3) {
+        // Stores staged through LDS one variable at a time: a ghost's variable
+        // is one aligned 64-byte line, written by four lanes with 16 bytes
+        // each, so a store instruction of the wave covers 16 whole lines
+        // instead of touching 64 lines with 16 bytes each.
+        __shared__ double2 sm[64 * 4];
+        const int lane = threadIdx.x;
+        int gq[4];
 #pragma unroll
-    for (int c = 0; c < ttd; c++)
-        #pragma unroll
-        for (int n = 0; n < 5; n++) p[n * ttd + c] = u2[c][n];
+        for (int q = 0; q < 4; q++) gq[q] = __shfl(g, (q * 64 + lane) >> 2);
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+            if (n) __syncthreads();
+#pragma unroll
+            for (int c = 0; c < 4; c++) sm[lane * 4 + c] = make_double2(u2[2 * c][n], u2[2 * c + 1][n]);
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (gq[q] >= 0) {
+                    double2 *p2 = reinterpret_cast<double2 *>(a.uold + (size_t)(a.first - 1 + gq[q]) * stride);
+                    p2[n * 4 + (lane & 3)] = sm[q * 64 + lane];
+                }
+        }
+    } else {
+        double *p = a.uold + (size_t)(a.first - 1 + g) * stride;
+#pragma unroll
+        for (int c = 0; c < ttd; c++)
+#pragma unroll
+            for (int n = 0; n < 5; n++) p[n * ttd + c] = u2[c][n];
+    }
 }
 
 hipError_t launch_refresh_ghosts(const GhostArgs &a, hipStream_t s)
 {
     if (a.nghost <= 0) return hipSuccess;
     if (a.ndim == 3)
-        hipLaunchKernelGGL(k_refresh_ghosts<3>, dim3((a.nghost + 127) / 128), dim3(128), 0, s, a);
+        hipLaunchKernelGGL(k_refresh_ghosts<3>, dim3((a.nghost + 63) / 64), dim3(64), 0, s, a);
     else
-        hipLaunchKernelGGL(k_refresh_ghosts<1>, dim3((a.nghost + 127) / 128), dim3(128), 0, s, a);
+        hipLaunchKernelGGL(k_refresh_ghosts<1>, dim3((a.nghost + 63) / 64), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1715,10 +1753,15 @@ __global__ void __launch_bounds__(256) k_upload_var(double *uold, const int32_t 
     const int io = (int)(i / 5), n = (int)(i % 5);
     const int igrid = father[2 * io], icell = father[2 * io + 1];
     if (igrid <= 0) return;
-    const double *c = uold + ((size_t)(head - 1 + io) * 5 + n) * 8;
+    // the oct's variable is one aligned 64-byte line: four 16-byte loads
+    const double2 *c = reinterpret_cast<const double2 *>(uold + ((size_t)(head - 1 + io) * 5 + n) * 8);
     double avg = 0.0;
 #pragma unroll
-    for (int k = 0; k < 8; k++) avg = avg + c[k];
+    for (int k = 0; k < 4; k++) {
+        const double2 v = c[k];
+        avg = avg + v.x;
+        avg = avg + v.y;
+    }
     uold[((size_t)(igrid - 1) * 5 + n) * 8 + (icell - 1)] = avg / 8.0;
 }
 

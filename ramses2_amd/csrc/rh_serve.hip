@@ -63,21 +63,22 @@ __global__ void k_init_ic(double *uold, double *unew, const int32_t *__restrict_
     }
 }
 
-// Device and pinned staging buffers of batched steps (shared by all contexts).
+// Device and pinned staging buffers of batched steps (shared by all contexts):
+// one block per step holding, 256-byte aligned, the job descriptors
+// ([2][njobs] GodArgs: parity 0 uold->unew, parity 1 unew->uold), the per-job
+// Courant records and the work items, uploaded with one copy.
 struct Batch {
-    int cap_jobs = 0, cap_items = 0;
-    GodArgs *d_descs = nullptr;    // [2][cap_jobs]: parity 0 (uold->unew) and 1 (unew->uold)
-    CourantJob *d_cjobs = nullptr; // [cap_jobs]
-    int2 *d_items = nullptr;       // [cap_items]
-    GodArgs *h_descs = nullptr;
-    CourantJob *h_cjobs = nullptr;
-    int2 *h_items = nullptr;
-    hipEvent_t staged = nullptr;   // the last upload from the staging buffers has completed
+    size_t cap = 0;
+    char *d_buf = nullptr;
+    char *h_buf = nullptr;
+    hipEvent_t staged = nullptr;   // the last upload from the staging buffer has completed
     bool pending = false;
 };
 Batch g_batch;
 
-int batch_reserve(int njobs, int nitems, hipStream_t s)
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+int batch_reserve(size_t bytes, hipStream_t s)
 {
     Batch &b = g_batch;
     if (!b.staged) {
@@ -87,27 +88,16 @@ int batch_reserve(int njobs, int nitems, hipStream_t s)
         RH_HIP_CHECK(hipEventSynchronize(b.staged));
         b.pending = false;
     }
-    if (njobs > b.cap_jobs) {
+    if (bytes > b.cap) {
+        const size_t cap = std::max(bytes, 2 * b.cap);
         RH_HIP_CHECK(hipStreamSynchronize(s));
-        (void)hipFree(b.d_descs);
-        (void)hipFree(b.d_cjobs);
-        (void)hipHostFree(b.h_descs);
-        (void)hipHostFree(b.h_cjobs);
-        const int cap = std::max(njobs, 2 * b.cap_jobs);
-        RH_HIP_CHECK(hipMalloc(&b.d_descs, 2 * (size_t)cap * sizeof(GodArgs)));
-        RH_HIP_CHECK(hipMalloc(&b.d_cjobs, (size_t)cap * sizeof(CourantJob)));
-        RH_HIP_CHECK(hipHostMalloc(&b.h_descs, 2 * (size_t)cap * sizeof(GodArgs), hipHostMallocDefault));
-        RH_HIP_CHECK(hipHostMalloc(&b.h_cjobs, (size_t)cap * sizeof(CourantJob), hipHostMallocDefault));
-        b.cap_jobs = cap;
-    }
-    if (nitems > b.cap_items) {
-        RH_HIP_CHECK(hipStreamSynchronize(s));
-        (void)hipFree(b.d_items);
-        (void)hipHostFree(b.h_items);
-        const int cap = std::max(nitems, 2 * b.cap_items);
-        RH_HIP_CHECK(hipMalloc(&b.d_items, (size_t)cap * sizeof(int2)));
-        RH_HIP_CHECK(hipHostMalloc(&b.h_items, (size_t)cap * sizeof(int2), hipHostMallocDefault));
-        b.cap_items = cap;
+        (void)hipFree(b.d_buf);
+        (void)hipHostFree(b.h_buf);
+        b.d_buf = b.h_buf = nullptr;
+        b.cap = 0;
+        RH_HIP_CHECK(hipMalloc(&b.d_buf, cap));
+        RH_HIP_CHECK(hipHostMalloc(&b.h_buf, cap, hipHostMallocDefault));
+        b.cap = cap;
     }
     return 0;
 }
@@ -357,8 +347,14 @@ int rh_batch_step(const int32_t *ids, int njobs, int precision, int nsteps)
     }
     if (nitems > (1l << 30)) return efail(RH_ERR_ARG, "too many bricks in one batch");
     hipStream_t stream = js[0]->stream;
-    if (int r = batch_reserve(njobs, (int)nitems, stream)) return r;
+    const size_t off_c = align256(2 * (size_t)njobs * sizeof(GodArgs));
+    const size_t off_i = off_c + align256((size_t)njobs * sizeof(CourantJob));
+    const size_t bytes = off_i + (size_t)nitems * sizeof(int2);
+    if (int r = batch_reserve(bytes, stream)) return r;
     Batch &b = g_batch;
+    GodArgs *h_descs = (GodArgs *)b.h_buf;
+    CourantJob *h_cjobs = (CourantJob *)(b.h_buf + off_c);
+    int2 *h_items = (int2 *)(b.h_buf + off_i);
     int pos = 0;
     for (int j = 0; j < njobs; j++) {
         State &s = *js[j];
@@ -366,29 +362,27 @@ int rh_batch_step(const int32_t *ids, int njobs, int precision, int nsteps)
         a.uold = s.uold;
         a.unew = s.unew;
         a.ubase = s.uold;
-        b.h_descs[j] = a;
+        h_descs[j] = a;
         a.uold = s.unew;
         a.unew = s.uold;
         a.ubase = s.unew;
-        b.h_descs[njobs + j] = a;
-        b.h_cjobs[j] = CourantJob{s.red_part, s.job.nbricks, s.job_dtmax, s.red_out, s.dt_dev, s.t_dev};
-        for (int i = 0; i < s.job.nbricks; i++) b.h_items[pos++] = make_int2(j, i);
+        h_descs[njobs + j] = a;
+        h_cjobs[j] = CourantJob{s.red_part, s.job.nbricks, s.job_dtmax, s.red_out, s.dt_dev, s.t_dev};
+        for (int i = 0; i < s.job.nbricks; i++) h_items[pos++] = make_int2(j, i);
     }
-    RH_HIP_CHECK(hipMemcpyAsync(b.d_descs, b.h_descs, 2 * (size_t)njobs * sizeof(GodArgs), hipMemcpyHostToDevice,
-                                stream));
-    RH_HIP_CHECK(hipMemcpyAsync(b.d_cjobs, b.h_cjobs, (size_t)njobs * sizeof(CourantJob), hipMemcpyHostToDevice,
-                                stream));
-    RH_HIP_CHECK(hipMemcpyAsync(b.d_items, b.h_items, (size_t)nitems * sizeof(int2), hipMemcpyHostToDevice, stream));
+    RH_HIP_CHECK(hipMemcpyAsync(b.d_buf, b.h_buf, bytes, hipMemcpyHostToDevice, stream));
     RH_HIP_CHECK(hipEventRecord(b.staged, stream));
     b.pending = true;
+    const GodArgs *d_descs = (const GodArgs *)b.d_buf;
+    const CourantJob *d_cjobs = (const CourantJob *)(b.d_buf + off_c);
     WorkList w;
-    w.items = b.d_items;
+    w.items = (const int2 *)(b.d_buf + off_i);
     for (int r = 0; r <= 8; r++) w.range[r] = (int)((nitems * r) / 8);
     for (int k = 0; k < nsteps; k++) {
-        w.descs = b.d_descs + (k & 1) * njobs;
+        w.descs = d_descs + (k & 1) * njobs;
         RH_HIP_CHECK(launch_godunov3d_wl(w, js[0]->job_bx, js[0]->job_by, js[0]->job_bz, js[0]->job.slope,
                                          js[0]->job.riemann, precision, stream, gen));
-        RH_HIP_CHECK(launch_courant_reduce_batch(b.d_cjobs, njobs, stream));
+        RH_HIP_CHECK(launch_courant_reduce_batch(d_cjobs, njobs, stream));
     }
     if (nsteps & 1)
         for (State *s : js) std::swap(s->uold, s->unew);

@@ -128,7 +128,7 @@ struct State {
     // ghost octs (interpolated missing neighbours), indices ngridmax+1..ngridmax+nghost_total
     int nghost_total = 0;
     int32_t *ghost_parent = nullptr;  // [ncachemax][2]
-    int32_t *ghost_src = nullptr;     // [ncachemax][7][2]
+    int32_t *ghost_src = nullptr;     // [7][2][ncachemax]: parent stencil of each ghost, by position in ghost_order
     int32_t *ghost_srckey = nullptr;  // [ncachemax][7][3]
     int32_t *ghost_order = nullptr;   // [ncachemax] ghost indices, each level's range in parent-cell order
     int32_t *ghost_key = nullptr;     // [ncachemax][3]
@@ -160,6 +160,9 @@ struct State {
     double job_dtmax = 0.0;
     AmrPlan amr;
 };
+
+// Leading dimension of the ghost tables laid out by ghost slot.
+inline int ghost_ld(const State &s) { return s.ncachemax > 0 ? s.ncachemax : 1; }
 
 // Godunov arguments of a 3D level of the current context (fused mode,
 // unew = uold + dF); lb receives the level's bricks.  Returns an RH_ERR code.
