@@ -419,7 +419,6 @@ static double choose_brick_offset(const int32_t *key, int n, const int32_t boxmi
             if (B[d] > 1 && B[d] / 2 <= lo[d] - boxmin[d]) c2.push_back(B[d] / 2);
             cand[d] = c2;
         }
-    const double pcell = 0.35 * (2 * B[0] + 4) * (2 * B[1] + 4), col = (double)(2 * B[0] + 2) * (2 * B[1] + 2);
     std::vector<int32_t> occ;
     double best = -1;
     for (int oz : cand[2])
@@ -442,8 +441,7 @@ static double choose_brick_offset(const int32_t *key, int n, const int32_t boxmi
                 double cost = 0;
                 for (int32_t v : occ)
                     if (v >= 0) {
-                        const double planes = 2.0 * ((v >> 16) - (v & 0xffff) + 1);
-                        cost += pcell * (planes + 4) + col * (planes + 2);
+                        cost += brick_work(B[0], B[1], 2.0 * ((v >> 16) - (v & 0xffff) + 1));
                     }
                 if (best < 0 || cost < best) {
                     best = cost;
@@ -957,6 +955,7 @@ int rh_build_nbor(int head_idx, int num_octs, int hash_size, int ckey_max_l, int
         RH_HIP_CHECK(dmalloc((void **)&lb.win, (size_t)lb.nbricks * nw * sizeof(int32_t)));
         RH_HIP_CHECK(hipMemcpyAsync(lb.org, org.data(), org.size() * sizeof(int32_t), hipMemcpyHostToDevice, s.stream));
         RH_HIP_CHECK(hipMemcpyAsync(lb.zr, zr.data(), zr.size() * sizeof(int32_t), hipMemcpyHostToDevice, s.stream));
+        lb.h_zr = zr;
         const long n = (long)lb.nbricks * nw;
         hipLaunchKernelGGL(k_build_bricks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s.stream, lb.org,
                            lb.nbricks, lb.bx, lb.by, lb.bz, c, lb.win);

@@ -95,7 +95,7 @@ hipError_t enqueue_amr_step(State &s, int parity, int precision)
         WorkList w;
         w.descs = m.d_descs + parity * m.ndesc;
         w.items = m.d_items + gr.first;
-        for (int r = 0; r <= 8; r++) w.range[r] = (int)(((long)gr.count * r) / 8);
+        for (int r = 0; r <= 8; r++) w.range[r] = gr.range[r];
         if ((e = launch_godunov3d_wl(w, gr.bx, gr.by, gr.bz, m.slope, m.riemann, precision, s.stream)) != hipSuccess)
             return e;
     }
@@ -138,6 +138,7 @@ int rh_amr_prepare(int levelmin, int levelmax, const int32_t *head, const int32_
     std::vector<GodArgs> descs;
     std::vector<int2> items;
     std::vector<std::array<int, 3>> shapes;  // brick shape of each level's descriptor
+    std::vector<const LevelBricks *> lbs;    // bricks of each level's descriptor
     const double gz[3] = {0, 0, 0};
     const double *gg = g ? g : gz;
     for (int l = levelmin; l <= levelmax; l++) {
@@ -154,6 +155,7 @@ int rh_amr_prepare(int levelmin, int levelmax, const int32_t *head, const int32_
             return r;
         a.reflux = l > levelmin ? 1 : 0;
         shapes.push_back({lb->bx, lb->by, lb->bz});
+        lbs.push_back(lb);
         descs.push_back(a);
         m.levels.push_back(l);
         m.dx_finest = dx;
@@ -170,6 +172,22 @@ int rh_amr_prepare(int levelmin, int levelmax, const int32_t *head, const int32_
             if (shapes[e] == shapes[d])
                 for (int b = 0; b < descs[e].nbricks; b++) items.push_back(make_int2((int)e, b));
         gr.count = (int)items.size() - gr.first;
+        // one range of the items per XCD, of equal modelled work: the bricks
+        // come in z-slab order, and a thin shell's bricks march fewer planes
+        // where it lies flat in z, so equal counts leave the XCDs unequal
+        std::vector<double> pre((size_t)gr.count + 1, 0.0);
+        for (int i = 0; i < gr.count; i++) {
+            const int2 it = items[(size_t)gr.first + i];
+            const LevelBricks &lb = *lbs[it.x];
+            const int zr = lb.h_zr.size() > (size_t)it.y ? lb.h_zr[it.y] : ((lb.bz - 1) << 16);
+            pre[i + 1] = pre[i] + brick_work(lb.bx, lb.by, 2.0 * ((zr >> 16) - (zr & 0xffff) + 1));
+        }
+        gr.range[0] = 0;
+        gr.range[8] = gr.count;
+        for (int r = 1; r < 8; r++) {
+            const int q = (int)(std::lower_bound(pre.begin(), pre.end(), pre.back() * r / 8.0) - pre.begin());
+            gr.range[r] = std::min(std::max(q, gr.range[r - 1]), gr.count);
+        }
         m.groups.push_back(gr);
     }
     m.noct_all = end - m.head0;

@@ -36,6 +36,7 @@ struct LevelBricks {
     int32_t *win = nullptr;   // [nbricks][bz+2][by+2][bx+2]
     int32_t *org = nullptr;   // [nbricks][3] oct key of the brick's first oct
     int32_t *zr = nullptr;    // [nbricks] z layers holding octs of the level: zlo | zhi << 16
+    std::vector<int32_t> h_zr;  // host copy of zr
     int any_ref = 1;          // some cell of the level's octs is refined
     int bnd = 1;              // some dimension is not periodic (boundary ghosts in the windows)
     int off[3] = {0, 0, 0};   // brick lattice offset: origins at boxmin + off + multiples of the shape
@@ -74,6 +75,7 @@ struct AmrPlan {
     struct Group {
         int bx, by, bz;
         int first, count;          // items [first, first+count) of d_items
+        int range[9];              // the items of XCD x: [range[x], range[x+1]), equal modelled work
     };
     std::vector<Group> groups;
     double *racc = nullptr;        // [max(nghost_total,1)][nslot][5] reflux sums per ghost oct
@@ -160,6 +162,16 @@ struct State {
     double job_dtmax = 0.0;
     AmrPlan amr;
 };
+
+// Modelled work of a brick of bx x by octs that marches `planes` cell planes
+// (its occupied z layers): per plane, plus two halo planes each side, the P
+// phase converts (2bx+4)(2by+4) cells and the T/F/U phases run (2bx+2)(2by+2)
+// columns, a P cell costing about a third of a column-plane.
+inline double brick_work(int bx, int by, double planes)
+{
+    const double pcell = 0.35 * (2 * bx + 4) * (2 * by + 4), col = (double)(2 * bx + 2) * (2 * by + 2);
+    return pcell * (planes + 4) + col * (planes + 2);
+}
 
 // Leading dimension of the ghost tables laid out by ghost slot.
 inline int ghost_ld(const State &s) { return s.ncachemax > 0 ? s.ncachemax : 1; }
