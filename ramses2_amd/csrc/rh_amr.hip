@@ -112,7 +112,7 @@ hipError_t enqueue_amr_step(State &s, int parity, int precision)
     }
     return launch_courant_ml(unext, s.refined, s.lev, m.head0, m.noct_all, m.boxlen, m.dx_finest, m.gamma, m.smallr,
                              __builtin_sqrt(m.smallc2), m.courant, m.g, s.red_part, s.red_cap, s.red_out, s.dt_dev,
-                             s.stream, s.t_dev);
+                             s.stream, s.t_dev, precision == 1);
 }
 
 }  // namespace

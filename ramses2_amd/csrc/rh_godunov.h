@@ -126,11 +126,12 @@ struct CourantJob {
 // courant_fine over the leaf cells of the contiguous oct range [head,
 // head+noct) spanning several levels: dx of each oct is boxlen*2^-lev.
 // dx_finest sets the initial dt bound dx/smallc.  Partials + final reduction
-// as launch_courant.
+// as launch_courant.  mixed: each cell's signal speed and dt in FP32 (the
+// mixed-precision steps); the sums and the min-reduction stay FP64.
 hipError_t launch_courant_ml(const double *uold, const uint8_t *refined, const int32_t *lev, int head, int noct,
                              double boxlen, double dx_finest, double gamma, double smallr, double smallc,
                              double courant_factor, const double g[3], double *partials, int max_blocks, double *out,
-                             double *dt_out, hipStream_t s, double *t_acc);
+                             double *dt_out, hipStream_t s, double *t_acc, bool mixed = false);
 // racc[0 .. 5*n) = 0 (a kernel: safe inside captured graphs)
 hipError_t launch_zero(double *p, long n, hipStream_t s);
 // unew(parent cell of ghost g) += the nslot accumulators of g (racc[5*nslot*g

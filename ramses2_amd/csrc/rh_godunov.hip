@@ -1359,7 +1359,10 @@ __global__ void __launch_bounds__(kCourantThreads) k_courant(const double *__res
 
 // courant_fine over octs of several levels: c0 carries the level-independent
 // constants (c0.dx = finest dx, c0.gsum_dx = sum |g|), dx of an oct is
-// boxlen*2^-lev (the same value the host computes as boxlen/2^lev).
+// boxlen*2^-lev (the same value the host computes as boxlen/2^lev).  MIXED:
+// the cell's signal speed and dt in FP32, as the mixed-precision kernels'
+// fused reduction (courant_cell_mixed).
+template <bool MIXED>
 __global__ void __launch_bounds__(kCourantThreads) k_courant_ml(const double *__restrict__ uold,
                                                                 const uint8_t *__restrict__ refined,
                                                                 const int32_t *__restrict__ lev, int head, int noct,
@@ -1380,7 +1383,10 @@ __global__ void __launch_bounds__(kCourantThreads) k_courant_ml(const double *__
         c.gsum_dx = c0.gsum_dx * dx;
         const double *p = uold + o * 40 + ind;
         const double u[5] = {p[0], p[8], p[16], p[24], p[32]};
-        courant_cell(u, c, gf0, mass, ekin, eint, dtmin);
+        if constexpr (MIXED)
+            courant_cell_mixed(u, c, mass, ekin, eint, dtmin);
+        else
+            courant_cell(u, c, gf0, mass, ekin, eint, dtmin);
     }
     __shared__ double red[kCourantThreads / 64][4];
     mass = wave_sum(mass);
@@ -2102,7 +2108,7 @@ hipError_t launch_courant_reduce_batch(const CourantJob *jobs, int njobs, hipStr
 hipError_t launch_courant_ml(const double *uold, const uint8_t *refined, const int32_t *lev, int head, int noct,
                              double boxlen, double dx_finest, double gamma, double smallr, double smallc,
                              double courant_factor, const double g[3], double *partials, int max_blocks, double *out,
-                             double *dt_out, hipStream_t s, double *t_acc)
+                             double *dt_out, hipStream_t s, double *t_acc, bool mixed)
 {
     CourantK c;
     c.dx = dx_finest;
@@ -2120,8 +2126,12 @@ hipError_t launch_courant_ml(const double *uold, const uint8_t *refined, const i
     int nb = (int)((ncell + kCourantThreads - 1) / kCourantThreads);
     nb = nb < max_blocks ? nb : max_blocks;
     nb = nb < 1 ? 1 : nb;
-    hipLaunchKernelGGL(k_courant_ml, dim3(nb), dim3(kCourantThreads), 0, s, uold, refined, lev, head, noct, c, boxlen,
-                       partials);
+    if (mixed)
+        hipLaunchKernelGGL(k_courant_ml<true>, dim3(nb), dim3(kCourantThreads), 0, s, uold, refined, lev, head, noct, c,
+                           boxlen, partials);
+    else
+        hipLaunchKernelGGL(k_courant_ml<false>, dim3(nb), dim3(kCourantThreads), 0, s, uold, refined, lev, head, noct,
+                           c, boxlen, partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_courant_final, dim3(1), dim3(256), 0, s, partials, nb, dx_finest / smallc, out, dt_out,

@@ -14,7 +14,7 @@ from ramses2_amd.autotune import DEFAULT_SHAPE
 from ramses2_amd.hydro import AmrHierarchy
 from tests.amr_cases import (leaf_total, oracle_mesh, oracle_step, restrict_all, smooth_state, three_level_mesh,
                              two_level_mesh)
-from tests.helpers import rel_err_per_cell, rel_err_per_var
+from tests.helpers import oracle_params, rel_err_per_cell, rel_err_per_var
 
 pytestmark = pytest.mark.gpu
 
@@ -109,6 +109,26 @@ def test_batched_levels_mixed_precision():
     gpu.run_device(1, precision=1)
     err = rel_err_per_var(gpu.state(), om.uold)
     assert np.all(err < 2e-6), err
+
+
+def test_batched_mixed_courant_in_fp32():
+    """The mixed batched step reduces each leaf cell's dt from an FP32 signal
+    speed: its next dt agrees with the oracle's FP64 courant_fine of the state
+    it leaves to 1e-5, and its leaf mass sum is the FP64 one."""
+    am = three_level_mesh()
+    u = smooth_state(am, 9, blast=True)
+    om = oracle_mesh(am, u, 0, 2)
+    restrict_all(am, om)
+    p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
+    gpu = AmrHierarchy(am, om.uold.copy(), p, interpol_var=0, interpol_type=2)
+    dt_next, t, sums, _ = gpu.run_device(2, precision=1)
+    u2 = gpu.state()
+    o2 = oracle_mesh(am, u2, 0, 2)
+    po = oracle_params(p)
+    dt_o = min(min(am.dx(l) / p.smallc, o2.courant_fine(l, po)[4]) for l in am.levels)
+    assert abs(dt_next - dt_o) <= 1e-5 * dt_o, (dt_next, dt_o)
+    mass = leaf_total(am, u2, 0)
+    assert abs(sums[0] - mass) <= 1e-12 * abs(mass), (sums[0], mass)
 
 
 def test_batched_levels_with_per_level_brick_shapes():
