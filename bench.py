@@ -57,7 +57,11 @@ def parse():
     ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs of the ensemble serving stream")
     ap.add_argument("--serve-load", type=float, default=0.8,
                     help="offered load as a fraction of the measured FP64 batched capacity")
+    ap.add_argument("--serve-rate", type=float, default=None,
+                    help="arrival rate in jobs/s (default: serve-load times the measured FP64 capacity)")
     ap.add_argument("--serve-steps", type=int, default=100)
+    ap.add_argument("--serve-trial-jobs", type=int, default=300,
+                    help="jobs of the stream's start served per candidate brick shape to pick each precision's shape")
     ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--no-serve", action="store_true")
@@ -68,6 +72,7 @@ def parse():
     ap.add_argument("--fp64-brick", type=str, default=None, help="brick shape bx,by,bz of the FP64 run (no tuning)")
     ap.add_argument("--mixed-brick", type=str, default=None, help="brick shape of the mixed run (no tuning)")
     ap.add_argument("--amr-brick", type=str, default=None, help="brick shape of every AMR level (no tuning)")
+    ap.add_argument("--serve-brick", type=str, default=None, help="brick shape of the serving jobs (no tuning)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -354,10 +359,14 @@ def amr_flags(h, am, p, err=0.1):
 
 def run_serving(args):
     """Ensemble serving: capacity of the batched FP64 step measured on a closed
-    batch of the stream's first jobs, then the Poisson stream of all jobs at
-    serve_load of that capacity, served in FP64 and in mixed precision."""
+    batch of the stream's first jobs (brick shape: the fastest tick of that
+    batch), then the Poisson stream of all jobs at serve_load of that
+    capacity, served in FP64 and in mixed precision, each with the brick
+    shape whose engine served the stream's first jobs with the lowest mean
+    latency (tune_serving)."""
     from ramses2_amd.autotune import DEFAULT_SHAPE
     from ramses2_amd import _lib
+    from ramses2_amd.autotune import tune_serving
     from ramses2_amd.serving import EnsembleServer, HipEngine, latency_summary, make_stream, mean_job_cells
 
     base = make_stream(args.serve_jobs, rate=1.0, seed=args.seed, nsteps=args.serve_steps)
@@ -367,7 +376,9 @@ def run_serving(args):
     cal = base[:args.serve_slots]
     # brick shape for the serving mix: batched FP64 ticks of the calibration jobs per shape
     shape_ms = {}
-    if not args.no_tune:
+    if args.serve_brick:
+        brick = tuple(int(x) for x in args.serve_brick.split(","))
+    elif not args.no_tune:
         from ramses2_amd.autotune import compiled_shapes
         for sh in compiled_shapes():
             e = HipEngine(precision=0, brick=sh)
@@ -424,14 +435,16 @@ def run_serving(args):
         engm.close(h)
     engm.close_all()
     mean_cells = mean_job_cells(base)
-    rate = args.serve_load * capacity / (mean_cells * args.serve_steps)
+    rate = args.serve_rate or args.serve_load * capacity / (mean_cells * args.serve_steps)
     for s in base:
         s.arrival = s.arrival / rate
     print("bench: serving capacity %.3g cell-updates/s FP64, %.3g mixed (%d jobs resident, batched steps); "
           "admission %.2f ms, release %.2f ms per job; rate %.2f jobs/s for load %.2f of FP64"
-          % (capacity, capacity_mixed, len(cal), t_open * 1e3, t_close * 1e3, rate, args.serve_load),
+          % (capacity, capacity_mixed, len(cal), t_open * 1e3, t_close * 1e3, rate,
+             rate * mean_cells * args.serve_steps / capacity),
           file=sys.stderr, flush=True)
-    out = {"jobs": len(base), "arrival_rate_jobs_per_s": rate, "offered_load": args.serve_load,
+    out = {"jobs": len(base), "arrival_rate_jobs_per_s": rate,
+           "offered_load": rate * mean_cells * args.serve_steps / capacity,
            "capacity_cell_updates_per_s": capacity, "capacity_mixed_cell_updates_per_s": capacity_mixed,
            "capacity_note": "%d jobs of the stream resident, %d batched steps timed in each precision; both "
            "precisions serve the stream at the FP64-derived rate" % (len(cal), ncal), "admission_ms_per_job": t_open * 1e3,
@@ -440,19 +453,35 @@ def run_serving(args):
            "max_slots": args.serve_slots, "hbm_budget_bytes": budget, "brick_shape": "%dx%dx%d" % brick,
            "brick_shape_ms_per_tick": {"%dx%dx%d" % k: v for k, v in shape_ms.items()} or None,
            "sizes": "N^3, N log-uniform in [32,128], even; 50% blast, 50% Riemann; gamma 1.4, MonCen, HLLC, C 0.5"}
+    eng.close_all()
     for prec, name in ((0, "fp64"), (1, "mixed")):
-        eng.precision = prec
-        srv = EnsembleServer(eng, max_slots=args.serve_slots, hbm_budget=budget)
+        # the shape for the stream itself: the shapes within 1.3x of the best
+        # capacity tick serve the stream's first jobs at its arrival times
+        shape_p, trial = brick, {}
+        if shape_ms and args.serve_trial_jobs > 0:
+            best_ms = min(shape_ms.values())
+            cands = [sh for sh, ms in sorted(shape_ms.items(), key=lambda x: x[1]) if ms <= 1.3 * best_ms]
+            shape_p, trial = tune_serving(base[:args.serve_trial_jobs], prec, cands, max_slots=args.serve_slots,
+                                          hbm_budget=budget)
+            print("bench: serving %s brick shapes (mean latency of the first %d jobs, s): %s -> %s"
+                  % (name, args.serve_trial_jobs, {"%dx%dx%d" % k: round(v, 4) for k, v in trial.items()}, shape_p),
+                  file=sys.stderr, flush=True)
+        engp = HipEngine(precision=prec, brick=shape_p)
+        for n in sorted({s.ncell for s in base}):
+            engp.warm(n)
+        srv = EnsembleServer(engp, max_slots=args.serve_slots, hbm_budget=budget)
         res = srv.run(base)
         summ = latency_summary(res, base)
         tr = np.array(srv.trace)
         summ.update({"ticks": srv.ticks, "mean_resident_jobs": float(tr[:, 1].mean()) if len(tr) else 0.0,
-                     "max_bytes_in_use": int(tr[:, 2].max()) if len(tr) else 0})
+                     "max_bytes_in_use": int(tr[:, 2].max()) if len(tr) else 0,
+                     "brick_shape": "%dx%dx%d" % shape_p,
+                     "brick_shape_trial_mean_latency_s": {"%dx%dx%d" % k: v for k, v in trial.items()} or None})
         out[name] = summ
         print("bench: serving %s p50 %.3f s p99 %.3f s, %.3g cell-updates/s over %.1f s"
               % (name, summ["p50_s"], summ["p99_s"], summ["throughput_cell_updates_per_s"], summ["span_s"]),
               file=sys.stderr, flush=True)
-    eng.close_all()
+        engp.close_all()
     _lib.call("rh_ctx_select", 0)
     return out
 

@@ -126,6 +126,30 @@ def tune_amr(amesh, uold, params, precision=0, steps=4, shapes=None, cache=DEFAU
     return bricks, times
 
 
+def tune_serving(specs, precision, shapes, max_slots=64, hbm_budget=96 << 30, params=None):
+    """Brick shape for serving a request stream at its arrival times: each
+    shape serves the same stream prefix (specs, arrival times in seconds) with
+    its own engine, and the shape with the lowest mean latency is kept.  A
+    tick's length depends on how well the resident jobs' bricks fill the GPU,
+    and the jobs resident under load are fewer than a capacity batch, so the
+    fastest shape for a full batch need not be the fastest under the stream.
+    Returns (best_shape, {shape: mean latency in s})."""
+    from .serving import EnsembleServer, HipEngine
+
+    lat = {}
+    for sh in shapes:
+        eng = HipEngine(params, precision=precision, brick=sh)
+        try:
+            for n in sorted({s.ncell for s in specs}):
+                eng.warm(n)
+            res = EnsembleServer(eng, max_slots=max_slots, hbm_budget=hbm_budget).run(specs)
+            lat[tuple(sh)] = float(np.mean([r.latency for r in res.values()]))
+        finally:
+            eng.close_all()
+    best = min(lat, key=lat.get)
+    return best, lat
+
+
 def brick_offset(keys, shape, box_min=(0, 0, 0), box_max=None):
     """The lattice offset rh_build_nbor gives a level's bricks of this shape
     (origins at box_min + offset + multiples of the shape) and its modelled
