@@ -60,6 +60,7 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="arrival rate in jobs/s (default: serve-load times the measured FP64 capacity)")
     ap.add_argument("--serve-steps", type=int, default=100)
+    ap.add_argument("--serve-depth", type=int, default=2, help="ticks the request loop keeps enqueued")
     ap.add_argument("--serve-trial-jobs", type=int, default=300,
                     help="jobs of the stream's start served per candidate brick shape to pick each precision's shape")
     ap.add_argument("--serve-slots", type=int, default=64)
@@ -450,7 +451,7 @@ def run_serving(args):
            "precisions serve the stream at the FP64-derived rate" % (len(cal), ncal), "admission_ms_per_job": t_open * 1e3,
            "release_ms_per_job": t_close * 1e3, "template_build_s": t_warm,
            "template_bytes": eng.template_bytes(), "mean_job_cells": mean_cells, "steps_per_job": args.serve_steps,
-           "max_slots": args.serve_slots, "hbm_budget_bytes": budget, "brick_shape": "%dx%dx%d" % brick,
+           "max_slots": args.serve_slots, "pipeline_depth_ticks": args.serve_depth, "hbm_budget_bytes": budget, "brick_shape": "%dx%dx%d" % brick,
            "brick_shape_ms_per_tick": {"%dx%dx%d" % k: v for k, v in shape_ms.items()} or None,
            "sizes": "N^3, N log-uniform in [32,128], even; 50% blast, 50% Riemann; gamma 1.4, MonCen, HLLC, C 0.5"}
     eng.close_all()
@@ -462,14 +463,14 @@ def run_serving(args):
             best_ms = min(shape_ms.values())
             cands = [sh for sh, ms in sorted(shape_ms.items(), key=lambda x: x[1]) if ms <= 1.3 * best_ms]
             shape_p, trial = tune_serving(base[:args.serve_trial_jobs], prec, cands, max_slots=args.serve_slots,
-                                          hbm_budget=budget)
+                                          hbm_budget=budget, depth=args.serve_depth)
             print("bench: serving %s brick shapes (mean latency of the first %d jobs, s): %s -> %s"
                   % (name, args.serve_trial_jobs, {"%dx%dx%d" % k: round(v, 4) for k, v in trial.items()}, shape_p),
                   file=sys.stderr, flush=True)
         engp = HipEngine(precision=prec, brick=shape_p)
         for n in sorted({s.ncell for s in base}):
             engp.warm(n)
-        srv = EnsembleServer(engp, max_slots=args.serve_slots, hbm_budget=budget)
+        srv = EnsembleServer(engp, max_slots=args.serve_slots, hbm_budget=budget, depth=args.serve_depth)
         res = srv.run(base)
         summ = latency_summary(res, base)
         tr = np.array(srv.trace)

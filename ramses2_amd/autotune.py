@@ -126,7 +126,7 @@ def tune_amr(amesh, uold, params, precision=0, steps=4, shapes=None, cache=DEFAU
     return bricks, times
 
 
-def tune_serving(specs, precision, shapes, max_slots=64, hbm_budget=96 << 30, params=None):
+def tune_serving(specs, precision, shapes, max_slots=64, hbm_budget=96 << 30, params=None, depth=2):
     """Brick shape for serving a request stream at its arrival times: each
     shape serves the same stream prefix (specs, arrival times in seconds) with
     its own engine, and the shape with the lowest mean latency is kept.  A
@@ -142,7 +142,7 @@ def tune_serving(specs, precision, shapes, max_slots=64, hbm_budget=96 << 30, pa
         try:
             for n in sorted({s.ncell for s in specs}):
                 eng.warm(n)
-            res = EnsembleServer(eng, max_slots=max_slots, hbm_budget=hbm_budget).run(specs)
+            res = EnsembleServer(eng, max_slots=max_slots, hbm_budget=hbm_budget, depth=depth).run(specs)
             lat[tuple(sh)] = float(np.mean([r.latency for r in res.values()]))
         finally:
             eng.close_all()

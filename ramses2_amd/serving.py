@@ -129,6 +129,7 @@ class HipEngine:
         self.precision = precision
         self._mesh = {}
         self._templates = {}
+        self._tbytes = None
         call("rh_init")
         if brick is None:
             v = [ctypes.c_int32() for _ in range(3)]
@@ -188,6 +189,7 @@ class HipEngine:
         if t is None:
             t = self._build(ncell)
             self._templates[ncell] = t
+            self._tbytes = None
         return t
 
     def warm(self, ncell):
@@ -195,7 +197,11 @@ class HipEngine:
         self._template(ncell)
 
     def template_bytes(self):
-        return sum(self.held_bytes(t) for t in self._templates.values())
+        """Device bytes of the template contexts (cached until a template is
+        built or destroyed: admission asks for it per job)."""
+        if self._tbytes is None:
+            self._tbytes = sum(self.held_bytes(t) for t in self._templates.values())
+        return self._tbytes
 
     def open(self, spec: JobSpec):
         """A new job context: the mesh cloned from the size's template (device
@@ -226,6 +232,7 @@ class HipEngine:
         for t in self._templates.values():
             call("rh_ctx_destroy", t)
         self._templates.clear()
+        self._tbytes = None
 
     def step(self, handles, nsteps=1):
         ids = np.ascontiguousarray(handles, dtype=np.int32)
@@ -307,8 +314,9 @@ class EnsembleServer:
     the head of the queue waits for room rather than being overtaken."""
 
     def __init__(self, engine, max_slots=64, hbm_budget=96 << 30, keep_states=False, clock=time.perf_counter,
-                 sleep=time.sleep):
+                 sleep=time.sleep, depth=2):
         self.engine = engine
+        self.depth = max(1, int(depth))   # ticks enqueued and not yet retired, at most
         self.max_slots = max_slots
         self.hbm_budget = hbm_budget
         self.keep_states = keep_states
@@ -320,11 +328,11 @@ class EnsembleServer:
         """Serve the stream; returns {job_id: JobResult}.  realtime=False admits
         every job at t=0 regardless of its arrival time (closed batch).
 
-        One tick stays in flight: the loop enqueues tick k (the jobs' step
-        counts are known on the host, so the jobs of tick k are too), then
-        waits for tick k-1 and retires the jobs that finished with it, then
-        admits new jobs, all while the device runs tick k.  A job admitted
-        during tick k joins tick k+1."""
+        Ticks stay in flight: the loop enqueues tick k (the jobs' step counts
+        are known on the host, so the jobs of tick k are too), then waits for
+        tick k-depth+1 and retires the jobs that finished with it, then admits
+        new jobs, all while the device runs the ticks still queued.  A job
+        admitted after tick k was enqueued joins tick k+1."""
         eng = self.engine
         pending = deque(sorted(specs, key=lambda s: (s.arrival, s.job_id)))
         queue = deque()
@@ -332,15 +340,14 @@ class EnsembleServer:
         resident = 0          # jobs holding device memory (active + finished, not yet retired)
         results = {}
         in_use = 0
-        inflight = None       # (fence, jobs that finish with the tick in flight)
+        inflight = deque()    # (fence, jobs that finish with that tick), oldest first
         t0 = self.clock()
         ticks = 0
         while pending or queue or active or inflight:
             now = self.clock() - t0
             while pending and (not realtime or pending[0].arrival <= now):
                 queue.append(pending.popleft())
-            # enqueue the next tick of every admitted job behind the one in flight
-            fence = None
+            # enqueue the next tick of every admitted job behind those in flight
             if active:
                 eng.step([a[1] for a in active])
                 ticks += 1
@@ -349,10 +356,11 @@ class EnsembleServer:
                     a[2] += 1
                     (done if a[2] >= a[0].nsteps else still).append(a)
                 active = still
-                fence = (eng.fence([a[1] for a in done]), done)
-            # the tick before: wait for it and retire the jobs it finished
-            if inflight:
-                f, done = inflight
+                inflight.append((eng.fence([a[1] for a in done]), done))
+            # the oldest tick beyond the depth (every tick once nothing is left
+            # to enqueue): wait for it and retire the jobs it finished
+            if inflight and (len(inflight) >= self.depth or not active):
+                f, done = inflight.popleft()
                 sts = eng.wait(f)
                 for a, st in zip(done, sts):
                     spec = a[0]
@@ -363,8 +371,7 @@ class EnsembleServer:
                     results[spec.job_id] = JobResult(spec.job_id, spec.ncell, spec.arrival if realtime else 0.0,
                                                      a[3], self.clock() - t0, st, state)
                 self.trace.append((self.clock() - t0, resident, in_use))
-            inflight = fence
-            # admissions overlap the tick just enqueued
+            # admissions overlap the ticks in flight
             while queue and resident < self.max_slots:
                 need = eng.footprint(queue[0])
                 reserved = eng.template_bytes()  # the per-size template meshes, resident beside the jobs

@@ -188,6 +188,40 @@ def test_one_tick_stays_in_flight():
     assert res[specs[0].job_id].status["steps"] == 3 and res[specs[1].job_id].status["steps"] == 2
 
 
+def test_deeper_pipeline_keeps_more_ticks_queued():
+    """depth=3: two ticks stay queued behind the one being retired; the jobs'
+    results are the same."""
+    specs = make_stream(2, rate=1.0, seed=8, nmin=8, nmax=16, nsteps=3)
+    specs[1].nsteps = 2
+    clock = FakeClock()
+    eng = CountingEngine(clock)
+    srv = EnsembleServer(eng, max_slots=64, hbm_budget=1 << 40, clock=clock, sleep=clock.sleep, depth=3)
+    res = srv.run(specs, realtime=False)
+    assert eng.log == [("open", 1), ("open", 2),
+                       ("step", 2), ("fence", 0),
+                       ("step", 2), ("fence", 1),
+                       ("step", 1), ("fence", 2), ("wait", 0),
+                       ("wait", 1), ("close", 2),
+                       ("wait", 2), ("close", 1)]
+    assert eng.max_outstanding == 3
+    assert res[specs[0].job_id].status["steps"] == 3 and res[specs[1].job_id].status["steps"] == 2
+
+
+def test_fifo_and_budget_hold_at_every_depth():
+    for depth in (1, 2, 4):
+        specs = make_stream(60, rate=400.0, seed=5, nmin=8, nmax=32, nsteps=7)
+        clock = FakeClock()
+        eng = CountingEngine(clock)
+        budget = 1000 * 3 * 32 ** 3
+        srv = EnsembleServer(eng, max_slots=5, hbm_budget=budget, clock=clock, sleep=clock.sleep, depth=depth)
+        res = srv.run(specs)
+        assert sorted(res) == [s.job_id for s in specs]
+        assert all(r.status["steps"] == 7 for r in res.values())
+        assert eng.max_open <= 5 and eng.max_bytes <= budget
+        assert eng.order == sorted(eng.order)
+        assert eng.max_outstanding <= depth and not eng.open_set
+
+
 def test_admission_counts_the_resident_templates():
     """The per-size template meshes stay on the device beside the jobs, so
     admission keeps jobs + templates within the HBM budget."""
