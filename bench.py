@@ -292,15 +292,23 @@ def run_amr(args):
         bricks, times = tune_amr(am, u0, p, precision=0, cache=None, key="sedov-amr")
     print("bench: AMR brick shapes (ms/step): %s -> %s" % ({k: round(v, 3) for k, v in times.items()}, bricks),
           file=sys.stderr, flush=True)
+    # mixed precision has its own best shape (its kernel has other register and LDS limits)
+    bricks_mixed, times_mixed = bricks, {}
+    if not args.amr_brick and not args.no_tune:
+        bricks_mixed, times_mixed = tune_amr(am, u0, p, precision=1, cache=None, key="sedov-amr")
+        print("bench: AMR mixed brick shapes (ms/step): %s -> %s"
+              % ({k: round(v, 3) for k, v in times_mixed.items()}, bricks_mixed), file=sys.stderr, flush=True)
     out = {"workload": "3D Sedov AMR, levels %d-%d, refinement on relative density/pressure gradient > 0.1 "
                        "(hydro_refine criterion on the analytic blast of radius 0.3), gamma 1.4, MonCen, HLLC, "
                        "Courant 0.5, periodic, nsubcycle=1" % (lmin, lmax),
            "octs_per_level": {str(l): int(am.noct[l]) for l in am.levels}, "cells": cells,
            "brick_shapes": {str(l): "%dx%dx%d" % b for l, b in bricks.items()},
-           "brick_shape_ms_per_step": times or None}
+           "brick_shape_ms_per_step": times or None,
+           "brick_shapes_mixed": {str(l): "%dx%dx%d" % b for l, b in bricks_mixed.items()},
+           "brick_shape_ms_per_step_mixed": times_mixed or None}
     states = {}
     for prec, name in ((0, "fp64"), (1, "mixed")):
-        h = AmrHierarchy(am, u0, p, bricks=bricks)
+        h = AmrHierarchy(am, u0, p, bricks=bricks if prec == 0 else bricks_mixed)
         h.run_device(max(args.warmup, 2), precision=prec, use_graph=True)
         dt, tt, sums, ms = h.run_device(args.amr_steps, precision=prec, use_graph=True)
         states[name] = (h.state(), sums)
