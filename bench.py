@@ -236,7 +236,13 @@ def sedov_uniform(args):
     p = HydroParams(gamma=1.4, courant_factor=0.5, slope_type=2, riemann="hllc")
     cells = ic.sedov_3d(n)
     ncells = mesh.noct * 8
-    HipHydro.set_brick_shape(*DEFAULT_SHAPE)
+    shape, shape_ms = DEFAULT_SHAPE, {}
+    if not args.no_tune:
+        from ramses2_amd.autotune import tune_uniform
+        shape, shape_ms = tune_uniform(mesh, cells, p, precision=0, cache=None)
+        print("bench: Sedov 128^3 brick shapes (Godunov ms): %s -> %s"
+              % ({"%dx%dx%d" % k: round(v, 3) for k, v in shape_ms.items()}, shape), file=sys.stderr, flush=True)
+    HipHydro.set_brick_shape(*shape)
     lev = UniformLevel(mesh, cells, p)
     lev.run_device(max(args.warmup, 2), use_graph=True)
     _lib.call("rh_device_sync")
@@ -251,7 +257,8 @@ def sedov_uniform(args):
            "gpu": {"cell_updates_per_s": ncells * args.sedov_steps / el, "ms_per_step": el / args.sedov_steps * 1e3,
                    "godunov_kernel_ms": kms,
                    "hbm_gbs_algorithmic": BYTES_PER_CELL * ncells / (kms * 1e-3) / 1e9 if kms else None,
-                   "brick_shape": "%dx%dx%d" % DEFAULT_SHAPE}}
+                   "brick_shape": "%dx%dx%d" % shape,
+                   "brick_shape_godunov_ms": {"%dx%dx%d" % k: v for k, v in shape_ms.items()} or None}}
     if not args.no_cpu:
         out["cpu"] = {}
         for threads in (len(os.sched_getaffinity(0)), 1):
@@ -459,7 +466,7 @@ def run_serving(args):
            "precisions serve the stream at the FP64-derived rate" % (len(cal), ncal), "admission_ms_per_job": t_open * 1e3,
            "release_ms_per_job": t_close * 1e3, "template_build_s": t_warm,
            "template_bytes": eng.template_bytes(), "mean_job_cells": mean_cells, "steps_per_job": args.serve_steps,
-           "max_slots": args.serve_slots, "pipeline_depth_ticks": args.serve_depth, "hbm_budget_bytes": budget, "brick_shape": "%dx%dx%d" % brick,
+           "max_slots": args.serve_slots, "pipeline_depth_ticks": args.serve_depth, "hbm_budget_bytes": budget, "capacity_brick_shape": "%dx%dx%d" % brick,
            "brick_shape_ms_per_tick": {"%dx%dx%d" % k: v for k, v in shape_ms.items()} or None,
            "sizes": "N^3, N log-uniform in [32,128], even; 50% blast, 50% Riemann; gamma 1.4, MonCen, HLLC, C 0.5"}
     eng.close_all()
